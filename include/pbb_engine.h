@@ -1,0 +1,195 @@
+/*
+ * pbb_engine.h -- C ABI of the MI355X IVM branch-and-bound engine for the
+ * permutation flow-shop problem (makespan objective).
+ *
+ * Drop-in boundary for three reference surfaces (jangmys/PBBPerm paths):
+ *   (i)   the lower-bound library    bounds/include/bound_abstract.h:6-34
+ *   (ii)  the GPU worker engine      gpu/include/gpubb.h:24-173 as driven by
+ *                                    distributed/src/worker_gpu.cpp:26-108
+ *   (iii) the factoradic work unit   distributed/include/fact_work.h:13-41
+ * and the gpuconfig.ini option surface (common/src/arguments.cpp:68-133).
+ *
+ * Conventions: every entry point returns 0 on success or a negative PBB_E*
+ * code (never exits the process); pbb_last_error() gives a message for the
+ * calling thread.  Processing-time matrices are m x n, machine-major int32
+ * (the stream layout of bounds/src/instance_flowshop.cpp:166-170).  One
+ * engine handle belongs to one host thread at a time; pbb_interrupt may be
+ * called from any thread.
+ */
+#ifndef PBB_ENGINE_H
+#define PBB_ENGINE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBB_OK 0
+#define PBB_E_ARG -1         /* invalid argument / unsupported option value */
+#define PBB_E_HIP -2         /* HIP runtime error */
+#define PBB_E_STATE -3       /* call not valid in the engine's current state */
+#define PBB_E_CAPACITY -4    /* buffer / device capacity exceeded */
+#define PBB_E_NODEVICE -5    /* no usable gfx950 device */
+
+#define PBB_MAX_JOBS 63      /* n <= 63: IVM row cells and stack entries 0..n fit one 64-lane wavefront */
+#define PBB_MAX_MACHINES 20  /* m <= 20 (the reference GPU cap, gpu_fsp_bound.cuh:12) */
+
+/* ------------------------------------------------------------------ config */
+
+/* Mirrors the [bb]/[gpu]/[initial] keys read by arguments::readIniFile. */
+typedef struct pbb_config {
+    int bound_mode;         /* [bb] boundingMode: 0 LB1, 1 strong only, 2 LB1 + refinement */
+    int branching;          /* [bb] adaptiveBranchingMode, CPU numbering:
+                               -1 front, 1 MaxSum, 2 MinBranch, 3 MinMin
+                               (ivm_bound.cpp:413-507) */
+    int refine;             /* refinement bound of modes 1/2: 0 = LB1' as shipped
+                               (ivm_bound.cpp:170), 1 = Johnson LB2 (ivm_bound.cpp:171) */
+    int johnson_pairs;      /* [bb] JohnsonMode: 0 all pairs, 1 (i,m), 2 (i,i+1) */
+    int early_stop_johnson; /* [bb] earlyStopJohnson (only prune-equivalent uses) */
+    int find_all;           /* [bb] findAll: prune with > instead of >= */
+    int single_node;        /* [bb] singleNode: MinBranch/MinMin use the live UB (1)
+                               or the UB at engine creation (0) */
+    int sorted_dfs;         /* [bb] sortedDFS: must be 0 */
+    int nb_ivms;            /* [gpu] nbIVMs: requested explorers (clamped to the
+                               device's resident-wavefront capacity) */
+    int steal_margin;       /* split intervals only at lines < n - steal_margin */
+    int init_mode;          /* [initial] ub: 0 from file, 1 heuristic (NEH), 3 infinity */
+    int init_solution;      /* [initial] sol: 0 identity root order, 1 NEH order */
+    int checkpoint_s;       /* [time] checkpoint */
+    int balance_s;          /* [time] balance: pbb_run returns after this period
+                               in worker mode */
+} pbb_config;
+
+/* Defaults of arguments.cpp:135-175 with sortedDFS forced to 0. */
+int pbb_config_default(pbb_config *cfg);
+
+/* Parse a gpuconfig.ini/bbconfig.ini file (same sections and keys as the
+ * reference).  problem/instance receive [problem] problem/instance (buffers of
+ * at least 64 bytes, may be NULL).  Unsupported values (sortedDFS != 0,
+ * JohnsonMode 3, branching -2/-3 with bound modes that read back bounds of
+ * the reference's stale arrays) return PBB_E_ARG. */
+int pbb_config_load_ini(const char *path, pbb_config *cfg, char *problem, char *instance);
+
+/* ------------------------------------------------------------- instances */
+
+/* Taillard instance ta<id> (bounds/src/instance_flowshop.cpp:146-181).
+ * ptm must hold m*n ints (at most 20*500). */
+int pbb_instance_taillard(int id, int *n, int *m, int32_t *ptm);
+
+/* Seeded synthetic instance: SplitMix64 stream, machine-major draws,
+ * p = 1 + (u mod 99) with rejection of the biased tail. */
+int pbb_instance_synth(uint64_t seed, int n, int m, int32_t *ptm);
+
+/* Shape draw for instance streams: n ~ U[nlo,nhi], m from mchoices. */
+int pbb_instance_synth_shape(uint64_t seed, int nlo, int nhi, const int *mchoices, int nchoices,
+                             int *n, int *m);
+
+/* VRF text instance (bounds/src/instance_vrf.cpp:31-69): "n m" then per job
+ * (machine, time) pairs.  Writes the machine-major PTM. */
+int pbb_instance_read_vrf(const char *path, int *n, int *m, int32_t *ptm, int cap);
+
+/* NEH heuristic (heuristic/src/fastNEH.cpp:151-243): perm in = start order,
+ * out = NEH order; returns the makespan in *cost. */
+int pbb_neh(const int32_t *ptm, int n, int m, int *perm, int *cost);
+
+/* Makespan of one permutation on the host (bound_fsp_weak::evalSolution). */
+int pbb_makespan_host(const int32_t *ptm, int n, int m, const int *perm, int *cost);
+
+/* ---------------------------------------------------------- bound library */
+
+/* Batched bound_abstract on device buffers (all pointers are device
+ * pointers; `stream` is a hipStream_t or NULL for the default stream).
+ * A partial schedule k is perms[k*n .. k*n+n-1] with front perms[0..l1[k]]
+ * and back perms[l2[k]..n-1], -1 <= l1 < l2 <= n. */
+typedef struct pbb_bounds pbb_bounds;
+
+/* bound_fsp_weak::init + bound_fsp_strong::init for one instance. */
+int pbb_bounds_create(const int32_t *ptm, int n, int m, pbb_bounds **out);
+int pbb_bounds_destroy(pbb_bounds *b);
+
+/* bound_fsp_weak::boundChildren (bounds/src/bound_fsp_weak.cpp:193-221):
+ * job-indexed cb/ce (count*n ints, 0 for non-children). both=0 -> front only
+ * (ce untouched). */
+int pbb_lb1_children_batch(pbb_bounds *b, int count, const int *perms, const int *l1,
+                           const int *l2, int both, int *cb, int *ce, void *stream);
+
+/* bound_fsp_weak::bornes_calculer (bounds/src/bound_fsp_weak.cpp:288-311). */
+int pbb_lb1_node_batch(pbb_bounds *b, int count, const int *perms, const int *l1, const int *l2,
+                       int *lb, void *stream);
+
+/* bound_fsp_strong::bornes_calculer with earlyExit=0, best=-1
+ * (bounds/src/bound_fsp_strong.cpp:540-560); pairs = JohnsonMode 0/1/2. */
+int pbb_lb2_node_batch(pbb_bounds *b, int count, const int *perms, const int *l1, const int *l2,
+                       int pairs, int *lb, void *stream);
+
+/* bound_fsp_weak::evalSolution over count full permutations. */
+int pbb_makespan_batch(pbb_bounds *b, int count, const int *perms, int *cmax, void *stream);
+
+/* ----------------------------------------------------------- the engine */
+
+typedef struct pbb_engine pbb_engine;
+
+typedef struct pbb_stats {
+    uint64_t decomposed;     /* pbab stats.totDecomposed (sequentialbb.cpp:129) */
+    uint64_t leaves;         /* stats.leaves (ivm_bound.cpp:243) */
+    uint64_t simple_bounds;  /* stats.simpleBounds (ivm_bound.cpp:225-229) */
+    uint64_t johnson_bounds; /* stats.johnsonBounds (ivm_bound.cpp:178) */
+    int explorers;           /* explorers actually running (resident wavefronts) */
+    int active;              /* explorers holding an interval after the last run */
+    double kernel_ms;        /* device time of the last pbb_run */
+    uint64_t depth_histo[PBB_MAX_JOBS + 2]; /* decomposed nodes per depth */
+} pbb_stats;
+
+/* gpubb(pbab*) + initialize + initializeBoundFSP + copyH2D
+ * (gpu/src/gpubb.cu:24-91, :974-1038).  ub is the incumbent cost (INT_MAX
+ * for none), ub_perm its permutation (may be NULL). */
+int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cfg, int ub,
+               const int *ub_perm, pbb_engine **out);
+int pbb_destroy(pbb_engine *e);
+
+/* Root order (root_sltn->perm, ivm_bound.cpp:288-293); identity by default. */
+int pbb_set_root(pbb_engine *e, const int *root_perm);
+
+/* gpubb::initFullInterval (gpu/src/gpubb.cu:95-154): bound the root with LB1
+ * and give the whole interval [0, n!-1] to explorer 0. Resets statistics. */
+int pbb_init_full(pbb_engine *e);
+
+/* gpubb::initFromFac (gpu/src/gpubb.cu:1244-1310): nb closed factoradic
+ * intervals pos[k*n..], end[k*n..] (digit i in [0, n-1-i]) for explorers
+ * ids[k] < explorers.  Each is unfolded (bounded, uncounted) on device. */
+int pbb_init_intervals(pbb_engine *e, int nb, const int *ids, const int *pos, const int *end);
+
+/* gpubb::next (gpu/src/gpubb.cu:272-306): explore until every explorer is
+ * idle (*all_done = 1), or until node_budget decomposed nodes (0 = none) /
+ * time_budget_s seconds (0 = none) / pbb_interrupt. */
+int pbb_run(pbb_engine *e, uint64_t node_budget, double time_budget_s, int *all_done);
+
+/* gpubb::getIntervals (gpu/src/gpubb.cu:1189-1241): export the interval of
+ * every active explorer; max must be >= explorers.  Node/leaf counters are
+ * returned in *decomposed / *leaves (may be NULL) and reset, as the
+ * reference moves them into the worker's work unit. */
+int pbb_get_intervals(pbb_engine *e, int *pos, int *end, int *ids, int *nb, int max,
+                      uint64_t *decomposed, uint64_t *leaves);
+
+/* Incumbent in/out (pbb->sltn, common/src/solution.cpp:26-88). set only
+ * lowers the cost (strict <), as solution::update. perm may be NULL. */
+int pbb_get_incumbent(pbb_engine *e, int *cost, int *perm);
+int pbb_set_incumbent(pbb_engine *e, int cost, const int *perm);
+
+int pbb_get_stats(pbb_engine *e, pbb_stats *st);
+
+/* gpubb::interruptExploration (gpu/src/gpubb.cu:156-162); honoured by the
+ * running kernel within a few thousand nodes. */
+int pbb_interrupt(pbb_engine *e);
+
+/* Explorer capacity the engine would use for (n, m, cfg) on this device. */
+int pbb_query_explorers(int n, int m, const pbb_config *cfg, int *explorers);
+
+const char *pbb_last_error(void);
+const char *pbb_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

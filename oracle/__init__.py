@@ -67,6 +67,7 @@ def lib():
         L.orc_bb_interval.argtypes = [C.c_void_p, IP, IP]
         L.orc_bb_share.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.orc_bb_depth_histo.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.orc_bounds_batch.argtypes = [C.c_void_p, C.c_int, IP, IP, IP, C.c_int, IP, IP, IP, IP, IP, IP]
         _lib = L
     return _lib
 
@@ -141,6 +142,22 @@ class Instance:
         idle = C.c_int(-7)
         v = lib().orc_lb2_node(self.h, _ip(p), l1, l2, pairs, C.byref(idle))
         return v, idle.value
+
+    def bounds_batch(self, perms, l1, l2, pairs: int = 0,
+                     which=("cb", "ce", "cbf", "lb1n", "lb2", "mk")):
+        """Bound values for arrays of partial schedules: dict of numpy arrays."""
+        perms = np.ascontiguousarray(perms, np.intc)
+        l1 = np.ascontiguousarray(l1, np.intc)
+        l2 = np.ascontiguousarray(l2, np.intc)
+        B = perms.shape[0]
+        out = {k: np.zeros((B, self.n), np.intc) for k in ("cb", "ce", "cbf") if k in which}
+        out.update({k: np.zeros(B, np.intc) for k in ("lb1n", "lb2", "mk") if k in which})
+        if "cb" in out and "ce" not in out:
+            out["ce"] = np.zeros((B, self.n), np.intc)
+        ptr = lambda k: _ip(out[k]) if k in out else None
+        lib().orc_bounds_batch(self.h, B, _ip(perms), _ip(l1), _ip(l2), pairs, ptr("cb"), ptr("ce"),
+                               ptr("cbf"), ptr("lb1n"), ptr("lb2"), ptr("mk"))
+        return out
 
     def neh(self) -> Tuple[int, np.ndarray]:
         p = np.arange(self.n, dtype=np.intc)
@@ -246,6 +263,31 @@ def solve(ptm: np.ndarray, ub: int, bound_mode=0, branching=1, refine=0, johnson
 
 
 # ---------------------------------------------------------------- reference harness
+
+def random_partials(n: int, count: int, seed: int, edge_frac: float = 0.01):
+    """Seeded random partial schedules (Fisher-Yates perm, l1 ~ U[-1, n-2],
+    l2 ~ U[l1+1, n]) with a fraction of edge cases (empty front, empty back,
+    l2-l1 in {1, 2})."""
+    rng = np.random.default_rng(seed)
+    perms = np.argsort(rng.random((count, n)), axis=1).astype(np.intc)
+    l1 = rng.integers(-1, n - 1, size=count).astype(np.intc)
+    l2 = (l1 + 1 + (rng.random(count) * (n - l1)).astype(np.intc)).astype(np.intc)
+    l2 = np.minimum(l2, n).astype(np.intc)
+    ne = max(4, int(count * edge_frac))
+    kinds = rng.integers(0, 4, size=ne)
+    idx = rng.choice(count, size=ne, replace=False)
+    for i, k in zip(idx, kinds):
+        if k == 0:
+            l1[i] = -1
+        elif k == 1:
+            l2[i] = n
+        elif k == 2:
+            l2[i] = l1[i] + 1
+        else:
+            l1[i] = min(l1[i], n - 2)
+            l2[i] = l1[i] + 2
+    return perms, l1, l2
+
 
 def ptm_text(ptm: np.ndarray) -> str:
     m, n = ptm.shape

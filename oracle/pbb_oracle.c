@@ -893,3 +893,24 @@ int orc_bb_share(orc_bb *V, orc_bb *T, int margin)
     T->line = l;
     return 1;
 }
+
+/* ======================================================================
+ * Batched checkers (the same functions over arrays of partial schedules)
+ * ====================================================================== */
+
+void orc_bounds_batch(const orc_inst *I, int count, const int *perms, const int *l1, const int *l2,
+                      int pairs_mode, int *cb, int *ce, int *cbf, int *lb1n, int *lb2, int *mk)
+{
+    int n = I->n;
+    for (int s = 0; s < count; s++) {
+        const int *p = perms + (size_t)s * n;
+        if (cb) orc_lb1_children(I, p, l1[s], l2[s], 1, cb + (size_t)s * n, ce + (size_t)s * n, NULL, NULL);
+        if (cbf) {
+            int dummy[ORC_MAXN];
+            orc_lb1_children(I, p, l1[s], l2[s], 0, cbf + (size_t)s * n, dummy, NULL, NULL);
+        }
+        if (lb1n) lb1n[s] = orc_lb1_node(I, p, l1[s], l2[s]);
+        if (lb2) lb2[s] = orc_lb2_node(I, p, l1[s], l2[s], pairs_mode, NULL);
+        if (mk) mk[s] = orc_makespan(I, p);
+    }
+}

@@ -120,6 +120,10 @@ int orc_bb_share(orc_bb *victim, orc_bb *thief, int min_depth_margin);
 /* Per-depth histogram of decomposed nodes (depth = line after goDown). */
 void orc_bb_depth_histo(const orc_bb *B, uint64_t *histo);
 
+/* Batched form of the bound functions above (NULL outputs are skipped). */
+void orc_bounds_batch(const orc_inst *I, int count, const int *perms, const int *l1, const int *l2,
+                      int pairs_mode, int *cb, int *ce, int *cbf, int *lb1n, int *lb2, int *mk);
+
 #ifdef __cplusplus
 }
 #endif

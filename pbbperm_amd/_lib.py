@@ -1,0 +1,118 @@
+"""ctypes binding of ``libpbb_engine.so`` (the C ABI of include/pbb_engine.h).
+
+The library is the only compute path: there is no Python or CPU fallback.
+If it is missing, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpbb_engine.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pbb_engine.h")
+
+PBB_MAX_JOBS = 63
+PBB_MAX_MACHINES = 20
+
+ERRORS = {-1: "PBB_E_ARG", -2: "PBB_E_HIP", -3: "PBB_E_STATE", -4: "PBB_E_CAPACITY", -5: "PBB_E_NODEVICE"}
+
+
+class PbbError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    """pbb_config: the [bb]/[gpu]/[initial]/[time] keys of gpuconfig.ini."""
+
+    _fields_ = [
+        ("bound_mode", C.c_int), ("branching", C.c_int), ("refine", C.c_int),
+        ("johnson_pairs", C.c_int), ("early_stop_johnson", C.c_int), ("find_all", C.c_int),
+        ("single_node", C.c_int), ("sorted_dfs", C.c_int), ("nb_ivms", C.c_int),
+        ("steal_margin", C.c_int), ("init_mode", C.c_int), ("init_solution", C.c_int),
+        ("checkpoint_s", C.c_int), ("balance_s", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("decomposed", C.c_uint64), ("leaves", C.c_uint64), ("simple_bounds", C.c_uint64),
+        ("johnson_bounds", C.c_uint64), ("explorers", C.c_int), ("active", C.c_int),
+        ("kernel_ms", C.c_double), ("depth_histo", C.c_uint64 * (PBB_MAX_JOBS + 2)),
+    ]
+
+
+IP = C.POINTER(C.c_int)
+I32P = C.POINTER(C.c_int32)
+U64P = C.POINTER(C.c_uint64)
+VP = C.c_void_p
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "pbb_config_default": (C.c_int, [C.POINTER(Config)]),
+    "pbb_config_load_ini": (C.c_int, [C.c_char_p, C.POINTER(Config), C.c_char_p, C.c_char_p]),
+    "pbb_instance_taillard": (C.c_int, [C.c_int, IP, IP, I32P]),
+    "pbb_instance_synth": (C.c_int, [C.c_uint64, C.c_int, C.c_int, I32P]),
+    "pbb_instance_synth_shape": (C.c_int, [C.c_uint64, C.c_int, C.c_int, IP, C.c_int, IP, IP]),
+    "pbb_instance_read_vrf": (C.c_int, [C.c_char_p, IP, IP, I32P, C.c_int]),
+    "pbb_neh": (C.c_int, [I32P, C.c_int, C.c_int, IP, IP]),
+    "pbb_makespan_host": (C.c_int, [I32P, C.c_int, C.c_int, IP, IP]),
+    "pbb_bounds_create": (C.c_int, [I32P, C.c_int, C.c_int, C.POINTER(VP)]),
+    "pbb_bounds_destroy": (C.c_int, [VP]),
+    "pbb_lb1_children_batch": (C.c_int, [VP, C.c_int, VP, VP, VP, C.c_int, VP, VP, VP]),
+    "pbb_lb1_node_batch": (C.c_int, [VP, C.c_int, VP, VP, VP, VP, VP]),
+    "pbb_lb2_node_batch": (C.c_int, [VP, C.c_int, VP, VP, VP, C.c_int, VP, VP]),
+    "pbb_makespan_batch": (C.c_int, [VP, C.c_int, VP, VP, VP]),
+    "pbb_create": (C.c_int, [I32P, C.c_int, C.c_int, C.POINTER(Config), C.c_int, IP, C.POINTER(VP)]),
+    "pbb_destroy": (C.c_int, [VP]),
+    "pbb_set_root": (C.c_int, [VP, IP]),
+    "pbb_init_full": (C.c_int, [VP]),
+    "pbb_init_intervals": (C.c_int, [VP, C.c_int, IP, IP, IP]),
+    "pbb_run": (C.c_int, [VP, C.c_uint64, C.c_double, IP]),
+    "pbb_get_intervals": (C.c_int, [VP, IP, IP, IP, IP, C.c_int, U64P, U64P]),
+    "pbb_get_incumbent": (C.c_int, [VP, IP, IP]),
+    "pbb_set_incumbent": (C.c_int, [VP, C.c_int, IP]),
+    "pbb_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
+    "pbb_interrupt": (C.c_int, [VP]),
+    "pbb_query_explorers": (C.c_int, [C.c_int, C.c_int, C.POINTER(Config), IP]),
+    "pbb_last_error": (C.c_char_p, []),
+    "pbb_version": (C.c_char_p, []),
+}
+
+_lib = None
+
+
+def load():
+    """Load libpbb_engine.so (raises if it has not been built).
+
+    torch is imported first: it carries its own HIP runtime
+    (libamdhip64.so, SONAME libamdhip64.so.7), and loading it before the
+    engine makes the engine's libamdhip64.so.7 dependency resolve to that
+    same runtime instead of a second copy from /opt/rocm, so device
+    pointers and streams are shared with torch.
+    """
+    global _lib
+    if _lib is None:
+        import torch  # noqa: F401  (HIP runtime shared with the engine)
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m pbbperm_amd.build` "
+                "(there is no fallback path)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> int:
+    if rc != 0:
+        raise PbbError(rc, load().pbb_last_error().decode())
+    return rc

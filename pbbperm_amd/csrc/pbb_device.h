@@ -1,0 +1,168 @@
+// pbb_device.h -- wave64 primitives and the per-wavefront bound routines of
+// the gfx950 IVM engine.  One IVM explorer = one 64-lane wavefront:
+//   * lane i holds row cell i of the current IVM row (job id, negative when
+//     pruned/visited, the flag encoding of common/include/misc.h:19-29);
+//   * lane l holds pos[l], end[l], dir[l] of the factoradic interval;
+//   * lane s holds entry s of three register stacks (one VGPR per machine):
+//       F[s] = completion times of the first s front jobs,
+//       B[s] = reverse completion times of the last s back jobs,
+//       R[s] = per-machine sum of the jobs still unscheduled at depth s,
+//     so the front/back/remain vectors of bound_fsp_weak::computePartial
+//     (bounds/src/bound_fsp_weak.cpp:61-130) are read with one v_readlane per
+//     machine instead of being recomputed from scratch at every node.
+// All arithmetic is int32 max-plus; the recurrences are those of the
+// reference, so every value is bit-identical.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define PBB_WAVE 64
+#define PBB_INF 0x7fffffff
+
+namespace pbb {
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// write a uniform value into lane l (v_cmp + v_cndmask)
+__device__ __forceinline__ int wrlane(int old, int v, int l)
+{
+    return __lane_id() == l ? v : old;
+}
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, PBB_WAVE); }
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PBB_WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, PBB_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, PBB_WAVE));
+    return v;
+}
+
+__device__ __forceinline__ int flag_neg(int j) { return j < 0 ? j : -j - 1; }  // misc.h negative()
+__device__ __forceinline__ int flag_abs(int j) { return j >= 0 ? j : -j - 1; }  // misc.h absolute()
+
+// Instance tables as seen by the device.
+struct DevInst {
+    int n, m;
+    int mp;               // padded stride of ptmT (multiple of 4)
+    const int *ptm;       // [m][n] machine-major
+    const int *ptmT;      // [n][mp] job-major
+    const int *min_arr;   // [m] bound_fsp_weak.cpp:270-283
+    const int *min_dep;   // [m] bound_fsp_weak.cpp:253-267
+    int npairs;
+    const int *pair_ab;   // [npairs] a | b << 8, (i<j) lexicographic (bound_fsp_strong.cpp:282-297)
+    const int4 *jtab;     // [npairs][n] {job, p_a, p_b, lag} in Johnson order
+};
+
+// ---------------------------------------------------------------------------
+// LB1 of one child per lane (bound_fsp_weak.cpp:133-191).
+// fv/bv/rv: uniform front/back/remain vectors.  pj: the lane's job's
+// processing times.  Writes the front-direction bound to lbF and, when both,
+// the back-direction bound to lbB; chF/chB receive the child's own front
+// (resp. back) completion chain (the values scheduleFront/scheduleBack would
+// compute with the child job appended), which the refinement bounds reuse.
+template <int M>
+__device__ __forceinline__ void lb1_child(const int (&fv)[M], const int (&bv)[M], const int (&rv)[M],
+                                          const int (&pj)[M], bool both, int &lbF, int &lbB,
+                                          int (&chF)[M], int (&chB)[M])
+{
+    int t0 = fv[0] + pj[0];
+    int lb = fv[0] + rv[0] + bv[0];
+    chF[0] = t0;
+#pragma unroll
+    for (int k = 1; k < M; k++) {
+        int t1 = max(fv[k], t0);
+        lb = max(lb, t1 + rv[k] + bv[k]);
+        t0 = t1 + pj[k];
+        chF[k] = t0;
+    }
+    lbF = lb;
+    if (both) {
+        t0 = bv[M - 1] + pj[M - 1];
+        lb = fv[M - 1] + rv[M - 1] + bv[M - 1];
+        chB[M - 1] = t0;
+#pragma unroll
+        for (int k = M - 2; k >= 0; k--) {
+            int t1 = max(t0, bv[k]);
+            lb = max(lb, fv[k] + rv[k] + t1);
+            t0 = t1 + pj[k];
+            chB[k] = t0;
+        }
+        lbB = lb;
+    } else {
+        lbB = 0;
+    }
+}
+
+// LB1' node bound (bound_fsp_weak.cpp:288-311) from explicit vectors.
+template <int M>
+__device__ __forceinline__ int lb1_node_vec(const int (&f)[M], const int (&r)[M], const int (&b)[M])
+{
+    int t0 = f[0] + r[0];
+    int lb = t0 + b[0];
+#pragma unroll
+    for (int k = 1; k < M; k++) {
+        int t1 = max(f[k] + r[k], t0);
+        lb = max(lb, t1 + b[k]);
+        t0 = t1;
+    }
+    return lb;
+}
+
+// Johnson two-machine bound over all machine pairs, one pair per lane
+// (bound_fsp_strong.cpp:50-100, :102-162 without early exit):
+// front/back are uniform vectors staged in LDS (fs/bs, M ints each),
+// unsched is the bitmask of unscheduled job ids (n <= 64).
+// pairs_mode: JohnsonMode 0 all, 1 (i, m-1), 2 (i, i+1).
+__device__ __forceinline__ int lb2_pairs_wave(const DevInst &I, const int *fs, const int *bs,
+                                              uint64_t unsched, int pairs_mode)
+{
+    const int lane = lane_id();
+    int best = 0;
+    for (int base = 0; base < I.npairs; base += PBB_WAVE) {
+        int p = base + lane;
+        if (p < I.npairs) {
+            int ab = I.pair_ab[p];
+            int a = ab & 0xff, b = ab >> 8;
+            bool use = true;
+            if (pairs_mode == 1 && b < I.m - 1) use = false;
+            if (pairs_mode == 2 && b != a + 1) use = false;
+            if (use) {
+                int t0 = fs[a], t1 = fs[b];
+                const int4 *row = I.jtab + (size_t)p * I.n;
+                for (int i = 0; i < I.n; i++) {
+                    int4 e = row[i];
+                    if ((unsched >> e.x) & 1ull) {
+                        t0 += e.y;
+                        t1 = max(t1, t0 + e.w) + e.z;
+                    }
+                }
+                best = max(best, max(t1 + bs[b], t0 + bs[a]));
+            }
+        }
+    }
+    return wave_max(best);
+}
+
+}  // namespace pbb

@@ -1,0 +1,944 @@
+// pbb_host.cpp -- host side of the MI355X IVM engine: the C ABI of
+// include/pbb_engine.h.  Instance tables are built here and uploaded once;
+// everything on the node path runs in the HIP kernels of pbb_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <climits>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/pbb_engine.h"
+#include "pbb_kernels.h"
+
+using namespace pbb;
+
+// ------------------------------------------------------------------ errors
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                      \
+    do {                                                                               \
+        hipError_t e_ = (x);                                                           \
+        if (e_ != hipSuccess)                                                          \
+            return fail(PBB_E_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                     \
+    } while (0)
+
+extern "C" const char *pbb_last_error(void) { return g_err.c_str(); }
+extern "C" const char *pbb_version(void) { return "pbbperm_amd 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------------ config
+extern "C" int pbb_config_default(pbb_config *c)
+{
+    if (!c) return fail(PBB_E_ARG, "null config");
+    c->bound_mode = 2;          // arguments.cpp:135
+    c->branching = 3;           // arguments.cpp:136
+    c->refine = 0;              // as shipped: ivm_bound.cpp:170
+    c->johnson_pairs = 1;       // arguments.cpp:158
+    c->early_stop_johnson = 1;  // arguments.cpp:157
+    c->find_all = 0;
+    c->single_node = 1;         // standalone solver (gpu/src/main.cpp:22)
+    c->sorted_dfs = 0;
+    c->nb_ivms = 4096;          // arguments.cpp:148
+    c->steal_margin = 6;        // gpu/include/loadbalance.cuh:90 (size_d - 6)
+    c->init_mode = -1;
+    c->init_solution = 0;
+    c->checkpoint_s = 1;
+    c->balance_s = 1;
+    return PBB_OK;
+}
+
+static int validate(const pbb_config *c)
+{
+    if (c->bound_mode < 0 || c->bound_mode > 2)
+        return fail(PBB_E_ARG, "boundingMode %d not in {0,1,2}", c->bound_mode);
+    if (!(c->branching == -3 || c->branching == -2 || c->branching == -1 || (c->branching >= 1 && c->branching <= 3)))
+        return fail(PBB_E_ARG, "adaptiveBranchingMode %d not in {-3,-2,-1,1,2,3}", c->branching);
+    if (c->refine != 0 && c->refine != 1) return fail(PBB_E_ARG, "refine must be 0 or 1");
+    if (c->johnson_pairs < 0 || c->johnson_pairs > 2)
+        return fail(PBB_E_ARG, "JohnsonMode %d unsupported (0, 1, 2; learned pairs are stateful)",
+                    c->johnson_pairs);
+    if (c->sorted_dfs != 0)
+        return fail(PBB_E_ARG, "sortedDFS=%d unsupported: the engine explores siblings in IVM row order (sortedDFS=0)",
+                    c->sorted_dfs);
+    if (c->nb_ivms < 1) return fail(PBB_E_ARG, "nbIVMs must be >= 1");
+    return PBB_OK;
+}
+
+static std::string trim(const std::string &s)
+{
+    size_t a = 0, b = s.size();
+    while (a < b && isspace((unsigned char)s[a])) a++;
+    while (b > a && isspace((unsigned char)s[b - 1])) b--;
+    return s.substr(a, b - a);
+}
+
+// INI reader with inih's conventions: '[section]', 'key = value', full-line
+// '#'/';' comments, inline ';' comments preceded by whitespace.
+static bool read_ini(const char *path, std::map<std::string, std::string> &kv)
+{
+    std::ifstream f(path);
+    if (!f) return false;
+    std::string line, section;
+    while (std::getline(f, line)) {
+        std::string t = trim(line);
+        if (t.empty() || t[0] == '#' || t[0] == ';') continue;
+        for (size_t i = 1; i < t.size(); i++)
+            if (t[i] == ';' && isspace((unsigned char)t[i - 1])) {
+                t = trim(t.substr(0, i));
+                break;
+            }
+        if (t[0] == '[') {
+            size_t r = t.find(']');
+            section = trim(t.substr(1, r == std::string::npos ? std::string::npos : r - 1));
+            continue;
+        }
+        size_t eq = t.find_first_of("=:");
+        if (eq == std::string::npos) continue;
+        std::string key = trim(t.substr(0, eq)), val = trim(t.substr(eq + 1));
+        std::string k = section + "." + key;
+        for (auto &ch : k) ch = (char)tolower((unsigned char)ch);
+        kv[k] = val;
+    }
+    return true;
+}
+
+static long ini_int(const std::map<std::string, std::string> &kv, const char *k, long dflt)
+{
+    auto it = kv.find(k);
+    if (it == kv.end()) return dflt;
+    char *endp = nullptr;
+    long v = strtol(it->second.c_str(), &endp, 0);
+    return (endp && *endp == '\0' && !it->second.empty()) ? v : dflt;
+}
+
+static bool ini_bool(const std::map<std::string, std::string> &kv, const char *k, bool dflt)
+{
+    auto it = kv.find(k);
+    if (it == kv.end()) return dflt;
+    std::string v = it->second;
+    for (auto &ch : v) ch = (char)tolower((unsigned char)ch);
+    if (v == "true" || v == "yes" || v == "on" || v == "1") return true;
+    if (v == "false" || v == "no" || v == "off" || v == "0") return false;
+    return dflt;
+}
+
+extern "C" int pbb_config_load_ini(const char *path, pbb_config *c, char *problem, char *instance)
+{
+    if (!path || !c) return fail(PBB_E_ARG, "null argument");
+    std::map<std::string, std::string> kv;
+    if (!read_ini(path, kv)) return fail(PBB_E_ARG, "cannot load %s", path);
+    pbb_config_default(c);
+    auto gs = [&](const char *k) {
+        auto it = kv.find(k);
+        return it == kv.end() ? std::string("UNKNOWN") : it->second;
+    };
+    if (problem) snprintf(problem, 64, "%s", gs("problem.problem").c_str());
+    if (instance) snprintf(instance, 64, "%s", gs("problem.instance").c_str());
+    // keys and defaults of arguments::readIniFile (common/src/arguments.cpp:68-133)
+    c->init_mode = (int)ini_int(kv, "initial.ub", -1);
+    c->init_solution = (int)ini_int(kv, "initial.sol", 0);
+    c->nb_ivms = (int)ini_int(kv, "gpu.nbivms", 4096);
+    c->sorted_dfs = (int)ini_int(kv, "bb.sorteddfs", 1);
+    c->early_stop_johnson = ini_bool(kv, "bb.earlystopjohnson", true) ? 1 : 0;
+    c->bound_mode = (int)ini_int(kv, "bb.boundingmode", 2);
+    c->johnson_pairs = (int)ini_int(kv, "bb.johnsonmode", 1);
+    c->find_all = ini_bool(kv, "bb.findall", false) ? 1 : 0;
+    c->single_node = ini_bool(kv, "bb.singlenode", false) ? 1 : 0;
+    c->branching = (int)ini_int(kv, "bb.adaptivebranchingmode", 3);
+    c->checkpoint_s = (int)ini_int(kv, "time.checkpoint", 1);
+    c->balance_s = (int)ini_int(kv, "time.balance", 1);
+    // engine-specific keys (not in the reference): refinement bound, split margin
+    c->refine = (int)ini_int(kv, "bb.refinementbound", 0);
+    c->steal_margin = (int)ini_int(kv, "gpu.stealmargin", 6);
+    return validate(c);
+}
+
+// --------------------------------------------------------------- instances
+static const long ta_seeds[120] = {
+    873654221, 379008056, 1866992158, 216771124, 495070989, 402959317, 1369363414, 2021925980,
+    573109518, 88325120, 587595453, 1401007982, 873136276, 268827376, 1634173168, 691823909,
+    73807235, 1273398721, 2065119309, 1672900551, 479340445, 268827376, 1958948863, 918272953,
+    555010963, 2010851491, 1519833303, 1748670931, 1923497586, 1829909967, 1328042058, 200382020,
+    496319842, 1203030903, 1730708564, 450926852, 1303135678, 1273398721, 587288402, 248421594,
+    1958948863, 575633267, 655816003, 1977864101, 93805469, 1803345551, 49612559, 1899802599,
+    2013025619, 578962478, 1539989115, 691823909, 655816003, 1315102446, 1949668355, 1923497586,
+    1805594913, 1861070898, 715643788, 464843328, 896678084, 1179439976, 1122278347, 416756875,
+    267829958, 1835213917, 1328833962, 1418570761, 161033112, 304212574, 1539989115, 655816003,
+    960914243, 1915696806, 2013025619, 1168140026, 1923497586, 167698528, 1528387973, 993794175,
+    450926852, 1462772409, 1021685265, 83696007, 508154254, 1861070898, 26482542, 444956424,
+    2115448041, 118254244, 471503978, 1215892992, 135346136, 1602504050, 160037322, 551454346,
+    519485142, 383947510, 1968171878, 540872513, 2013025619, 475051709, 914834335, 810642687,
+    1019331795, 2056065863, 1342855162, 1325809384, 1988803007, 765656702, 1368624604, 450181436,
+    1927888393, 1759567256, 606425239, 19268348, 1298201670, 2041736264, 379756761, 28837162};
+
+extern "C" int pbb_instance_taillard(int id, int *n_out, int *m_out, int32_t *ptm)
+{
+    if (id < 1 || id > 120 || !n_out || !m_out || !ptm) return fail(PBB_E_ARG, "taillard id %d", id);
+    // shapes of instance_flowshop.cpp:104-144
+    int n = id > 110 ? 500 : id > 90 ? 200 : id > 60 ? 100 : id > 30 ? 50 : 20;
+    static const int lim[] = {110, 100, 90, 80, 70, 60, 50, 40, 30, 20, 10};
+    static const int val[] = {20, 20, 10, 20, 10, 5, 20, 10, 5, 20, 10};
+    int m = 5;
+    for (int i = 0; i < 11; i++)
+        if (id > lim[i]) {
+            m = val[i];
+            break;
+        }
+    // Park-Miller via Schrage, single-precision scaling (instance_flowshop.cpp:146-157)
+    long seed = ta_seeds[id - 1];
+    for (int k = 0; k < m; k++)
+        for (int j = 0; j < n; j++) {
+            const long M = 2147483647, A = 16807, Q = 127773, R = 2836;
+            long q = seed / Q;
+            seed = A * (seed % Q) - q * R;
+            if (seed < 0) seed += M;
+            float u = (float)seed / (float)M;
+            ptm[k * n + j] = (int32_t)(1 + (long)((double)u * 99.0));
+        }
+    *n_out = n;
+    *m_out = m;
+    return PBB_OK;
+}
+
+static uint64_t sm_next(uint64_t *s)
+{
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static uint64_t sm_uniform(uint64_t *s, uint64_t k)
+{
+    uint64_t limit = UINT64_MAX - (UINT64_MAX % k + 1) % k;
+    uint64_t x;
+    do x = sm_next(s);
+    while (x > limit);
+    return x % k;
+}
+
+extern "C" int pbb_instance_synth(uint64_t seed, int n, int m, int32_t *ptm)
+{
+    if (n < 2 || m < 2 || !ptm) return fail(PBB_E_ARG, "bad synth shape");
+    uint64_t s = seed;
+    for (int k = 0; k < m; k++)
+        for (int j = 0; j < n; j++) ptm[k * n + j] = 1 + (int32_t)sm_uniform(&s, 99);
+    return PBB_OK;
+}
+
+extern "C" int pbb_instance_synth_shape(uint64_t seed, int nlo, int nhi, const int *mchoices,
+                                        int nchoices, int *n, int *m)
+{
+    if (nhi < nlo || nchoices < 1 || !mchoices || !n || !m) return fail(PBB_E_ARG, "bad shape range");
+    uint64_t s = seed ^ 0x5348415045ull;
+    *n = nlo + (int)sm_uniform(&s, (uint64_t)(nhi - nlo + 1));
+    *m = mchoices[sm_uniform(&s, (uint64_t)nchoices)];
+    return PBB_OK;
+}
+
+extern "C" int pbb_instance_read_vrf(const char *path, int *n_out, int *m_out, int32_t *ptm, int cap)
+{
+    std::ifstream f(path);
+    if (!f) return fail(PBB_E_ARG, "cannot open %s", path);
+    int n = 0, m = 0;
+    f >> n >> m;
+    if (n < 1 || m < 1) return fail(PBB_E_ARG, "bad VRF header in %s", path);
+    if (n * m > cap) return fail(PBB_E_CAPACITY, "ptm buffer too small (%d < %d)", cap, n * m);
+    // job-major (machine, time) pairs, transposed on read (instance_vrf.cpp:50-64)
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < m; k++) {
+            int mach = 0, t = 0;
+            if (!(f >> mach >> t)) return fail(PBB_E_ARG, "truncated VRF file %s", path);
+            ptm[k * n + j] = t;
+        }
+    *n_out = n;
+    *m_out = m;
+    return PBB_OK;
+}
+
+static int host_partial_makespan(const int32_t *ptm, int n, int m, const int *seq, int len)
+{
+    int c[PBB_MAX_MACHINES * 4] = {0};
+    std::vector<int> cv(m, 0);
+    (void)c;
+    for (int i = 0; i < len; i++) {
+        int j = seq[i];
+        cv[0] += ptm[j];
+        for (int k = 1; k < m; k++) cv[k] = std::max(cv[k - 1], cv[k]) + ptm[k * n + j];
+    }
+    return cv[m - 1];
+}
+
+extern "C" int pbb_makespan_host(const int32_t *ptm, int n, int m, const int *perm, int *cost)
+{
+    if (!ptm || !perm || !cost || n < 1 || m < 1) return fail(PBB_E_ARG, "bad makespan args");
+    *cost = host_partial_makespan(ptm, n, m, perm, n);
+    return PBB_OK;
+}
+
+// NEH (fastNEH.cpp:151-243): stable sort by decreasing total time, the first
+// two jobs kept in sorted order only if strictly better, then insertion of
+// each job at the first position of minimum partial makespan.
+extern "C" int pbb_neh(const int32_t *ptm, int n, int m, int *perm, int *cost)
+{
+    if (!ptm || !perm || !cost || n < 2 || m < 1) return fail(PBB_E_ARG, "bad NEH args");
+    std::vector<int> tot(n, 0);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < m; k++) tot[j] += ptm[k * n + j];
+    std::stable_sort(perm, perm + n, [&](int a, int b) { return tot[a] > tot[b]; });
+    int two[2] = {perm[1], perm[0]};
+    if (!(host_partial_makespan(ptm, n, m, perm, 2) < host_partial_makespan(ptm, n, m, two, 2))) {
+        perm[0] = two[0];
+        perm[1] = two[1];
+    }
+    std::vector<int> tmp(n);
+    for (int k = 2; k < n; k++) {
+        int job = perm[k], bestc = INT_MAX, bestp = 0;
+        for (int p = 0; p <= k; p++) {
+            int t = 0;
+            for (int i = 0; i < p; i++) tmp[t++] = perm[i];
+            tmp[t++] = job;
+            for (int i = p; i < k; i++) tmp[t++] = perm[i];
+            int c = host_partial_makespan(ptm, n, m, tmp.data(), k + 1);
+            if (c < bestc) {
+                bestc = c;
+                bestp = p;
+            }
+        }
+        for (int p = k; p > bestp; p--) perm[p] = perm[p - 1];
+        perm[bestp] = job;
+    }
+    *cost = host_partial_makespan(ptm, n, m, perm, n);
+    return PBB_OK;
+}
+
+// ------------------------------------------------------------ device tables
+struct HostTables {
+    int n, m, mp, npairs;
+    std::vector<int> ptm, ptmT, min_arr, min_dep, pair_ab;
+    std::vector<int4> jtab;
+};
+
+static void build_tables(const int32_t *ptm, int n, int m, HostTables &T)
+{
+    T.n = n;
+    T.m = m;
+    T.mp = (m + 3) & ~3;
+    T.ptm.assign(ptm, ptm + (size_t)n * m);
+    T.ptmT.assign((size_t)n * T.mp, 0);
+    for (int j = 0; j < n; j++)
+        for (int k = 0; k < m; k++) T.ptmT[(size_t)j * T.mp + k] = ptm[k * n + j];
+    // bound_fsp_weak::fillMinTimeArrDep (bound_fsp_weak.cpp:248-286)
+    T.min_arr.assign(m, INT_MAX);
+    T.min_dep.assign(m, INT_MAX);
+    T.min_arr[0] = 0;
+    T.min_dep[m - 1] = 0;
+    for (int j = 0; j < n; j++) {
+        int tail = 0;
+        for (int k = m - 2; k >= 0; k--) {
+            tail += ptm[(k + 1) * n + j];
+            T.min_dep[k] = std::min(T.min_dep[k], tail);
+        }
+        int head = 0;
+        for (int k = 1; k < m; k++) {
+            head += ptm[(k - 1) * n + j];
+            T.min_arr[k] = std::min(T.min_arr[k], head);
+        }
+    }
+    // machine pairs and Johnson/Mitten orders (bound_fsp_strong.cpp:264-436).
+    // Ties inside a key leave the two-machine optimum unchanged; a stable
+    // order by (set, key) is used.
+    T.npairs = m * (m - 1) / 2;
+    T.pair_ab.clear();
+    T.jtab.assign((size_t)T.npairs * n, int4{0, 0, 0, 0});
+    int p = 0;
+    for (int a = 0; a < m - 1; a++)
+        for (int b = a + 1; b < m; b++, p++) {
+            T.pair_ab.push_back(a | (b << 8));
+            std::vector<int> lag(n), set(n), key(n), ord(n);
+            for (int j = 0; j < n; j++) {
+                int l = 0;
+                for (int k = a + 1; k < b; k++) l += ptm[k * n + j];
+                lag[j] = l;
+                bool first = ptm[a * n + j] < ptm[b * n + j];
+                set[j] = first ? 1 : 2;
+                key[j] = (first ? ptm[a * n + j] : ptm[b * n + j]) + l;
+                ord[j] = j;
+            }
+            std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+                if (set[x] != set[y]) return set[x] < set[y];
+                return set[x] == 1 ? key[x] < key[y] : key[x] > key[y];
+            });
+            for (int i = 0; i < n; i++) {
+                int j = ord[i];
+                T.jtab[(size_t)p * n + i] = int4{j, ptm[a * n + j], ptm[b * n + j], lag[j]};
+            }
+        }
+}
+
+struct DevTables {
+    int *ptm = nullptr, *ptmT = nullptr, *min_arr = nullptr, *min_dep = nullptr, *pair_ab = nullptr;
+    int4 *jtab = nullptr;
+    DevInst I{};
+};
+
+static int upload_tables(const HostTables &T, DevTables &D)
+{
+    HIPCHK(hipMalloc(&D.ptm, T.ptm.size() * 4));
+    HIPCHK(hipMalloc(&D.ptmT, T.ptmT.size() * 4));
+    HIPCHK(hipMalloc(&D.min_arr, T.m * 4));
+    HIPCHK(hipMalloc(&D.min_dep, T.m * 4));
+    HIPCHK(hipMalloc(&D.pair_ab, std::max(1, T.npairs) * 4));
+    HIPCHK(hipMalloc(&D.jtab, std::max<size_t>(1, T.jtab.size()) * sizeof(int4)));
+    HIPCHK(hipMemcpy(D.ptm, T.ptm.data(), T.ptm.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.ptmT, T.ptmT.data(), T.ptmT.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.min_arr, T.min_arr.data(), T.m * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(D.min_dep, T.min_dep.data(), T.m * 4, hipMemcpyHostToDevice));
+    if (T.npairs) {
+        HIPCHK(hipMemcpy(D.pair_ab, T.pair_ab.data(), T.npairs * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(D.jtab, T.jtab.data(), T.jtab.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    D.I.n = T.n;
+    D.I.m = T.m;
+    D.I.mp = T.mp;
+    D.I.ptm = D.ptm;
+    D.I.ptmT = D.ptmT;
+    D.I.min_arr = D.min_arr;
+    D.I.min_dep = D.min_dep;
+    D.I.npairs = T.npairs;
+    D.I.pair_ab = D.pair_ab;
+    D.I.jtab = D.jtab;
+    return PBB_OK;
+}
+
+static void free_tables(DevTables &D)
+{
+    hipFree(D.ptm);
+    hipFree(D.ptmT);
+    hipFree(D.min_arr);
+    hipFree(D.min_dep);
+    hipFree(D.pair_ab);
+    hipFree(D.jtab);
+    D = DevTables{};
+}
+
+static int check_shape(const int32_t *ptm, int n, int m)
+{
+    if (!ptm) return fail(PBB_E_ARG, "null ptm");
+    if (n < 3 || n > PBB_MAX_JOBS) return fail(PBB_E_ARG, "n=%d outside [3, %d]", n, PBB_MAX_JOBS);
+    if (m < 2 || m > PBB_MAX_MACHINES)
+        return fail(PBB_E_ARG, "m=%d outside [2, %d]", m, PBB_MAX_MACHINES);
+    for (int i = 0; i < n * m; i++)
+        if (ptm[i] < 0 || ptm[i] > 1000000) return fail(PBB_E_ARG, "processing time %d out of range", ptm[i]);
+    return PBB_OK;
+}
+
+static int check_device()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count < 1) return fail(PBB_E_NODEVICE, "no HIP device available");
+    return PBB_OK;
+}
+
+// ------------------------------------------------------------ bound library
+struct pbb_bounds {
+    HostTables T;
+    DevTables D;
+};
+
+extern "C" int pbb_bounds_create(const int32_t *ptm, int n, int m, pbb_bounds **out)
+{
+    if (!out) return fail(PBB_E_ARG, "null out");
+    int rc = check_shape(ptm, n, m);
+    if (rc) return rc;
+    if ((rc = check_device())) return rc;
+    pbb_bounds *b = new pbb_bounds();
+    build_tables(ptm, n, m, b->T);
+    rc = upload_tables(b->T, b->D);
+    if (rc) {
+        free_tables(b->D);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return PBB_OK;
+}
+
+extern "C" int pbb_bounds_destroy(pbb_bounds *b)
+{
+    if (!b) return PBB_OK;
+    free_tables(b->D);
+    delete b;
+    return PBB_OK;
+}
+
+static int bounds_call(pbb_bounds *b, int which, int count, const int *perms, const int *l1,
+                       const int *l2, int arg, int *o1, int *o2, void *stream)
+{
+    if (!b || count < 0 || !perms || !o1) return fail(PBB_E_ARG, "bad bound batch arguments");
+    if (which != 3 && (!l1 || !l2)) return fail(PBB_E_ARG, "null l1/l2");
+    if (which == 0 && arg && !o2) return fail(PBB_E_ARG, "null ce with both=1");
+    if (count == 0) return PBB_OK;
+    KArgs a{};
+    a.I = b->D.I;
+    HIPCHK(launch_bounds(which, a, count, perms, l1, l2, arg, o1, o2, (hipStream_t)stream));
+    return PBB_OK;
+}
+
+extern "C" int pbb_lb1_children_batch(pbb_bounds *b, int count, const int *perms, const int *l1,
+                                      const int *l2, int both, int *cb, int *ce, void *stream)
+{
+    return bounds_call(b, 0, count, perms, l1, l2, both ? 1 : 0, cb, ce, stream);
+}
+
+extern "C" int pbb_lb1_node_batch(pbb_bounds *b, int count, const int *perms, const int *l1,
+                                  const int *l2, int *lb, void *stream)
+{
+    return bounds_call(b, 1, count, perms, l1, l2, 0, lb, nullptr, stream);
+}
+
+extern "C" int pbb_lb2_node_batch(pbb_bounds *b, int count, const int *perms, const int *l1,
+                                  const int *l2, int pairs, int *lb, void *stream)
+{
+    if (pairs < 0 || pairs > 2) return fail(PBB_E_ARG, "pairs mode %d", pairs);
+    return bounds_call(b, 2, count, perms, l1, l2, pairs, lb, nullptr, stream);
+}
+
+extern "C" int pbb_makespan_batch(pbb_bounds *b, int count, const int *perms, int *cmax, void *stream)
+{
+    return bounds_call(b, 3, count, perms, nullptr, nullptr, 0, cmax, nullptr, stream);
+}
+
+// ------------------------------------------------------------------ engine
+struct pbb_engine {
+    pbb_config cfg;
+    HostTables T;
+    DevTables D;
+    int n, m, E, blocks, ring_cap;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // device buffers
+    int8_t *mat = nullptr, *pos = nullptr, *end = nullptr, *dir = nullptr;
+    int *line = nullptr, *state = nullptr, *ring_id = nullptr;
+    unsigned *ring_seq = nullptr;
+    Ctl *ctl = nullptr;
+    unsigned long long *best = nullptr;
+    int *best_perm = nullptr, *root_row = nullptr, *root_dir = nullptr, *root_perm_d = nullptr;
+    int *host_flag = nullptr, *host_flag_d = nullptr;
+    // host state
+    std::vector<int> root_perm;
+    int inc_cost;
+    std::vector<int> inc_perm;
+    int initial_ub;
+    bool root_bounded = false;
+    bool initialized = false;
+    unsigned long long exported_dec = 0, exported_leaves = 0;
+    double last_ms = 0.0;
+    int wall_khz = 100000;
+};
+
+static KArgs make_args(pbb_engine *e)
+{
+    KArgs a{};
+    a.I = e->D.I;
+    a.E = e->E;
+    a.mat = e->mat;
+    a.pos = e->pos;
+    a.end = e->end;
+    a.dir = e->dir;
+    a.line = e->line;
+    a.state = e->state;
+    a.ring_id = e->ring_id;
+    a.ring_seq = e->ring_seq;
+    a.ring_cap = e->ring_cap;
+    a.ctl = e->ctl;
+    a.best = e->best;
+    a.best_perm = e->best_perm;
+    a.root_row = e->root_row;
+    a.root_dir = e->root_dir;
+    a.root_row_out = e->root_row;
+    a.root_dir_out = e->root_dir;
+    a.host_flag = e->host_flag_d;
+    a.bound_mode = e->cfg.bound_mode;
+    a.branching = e->cfg.branching;
+    a.refine = e->cfg.refine;
+    a.pairs_mode = e->cfg.johnson_pairs;
+    a.find_all = e->cfg.find_all;
+    a.single_node = e->cfg.single_node;
+    a.initial_ub = e->initial_ub;
+    a.steal_margin = e->cfg.steal_margin;
+    return a;
+}
+
+static int capacity(pbb_engine *e, int *explorers)
+{
+    KArgs a = make_args(e);
+    int bpc = 0;
+    HIPCHK(explore_occupancy(a, &bpc));
+    if (bpc < 1) return fail(PBB_E_CAPACITY, "explorer kernel does not fit one workgroup per CU (LDS)");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    *explorers = bpc * prop.multiProcessorCount * 4;
+    return PBB_OK;
+}
+
+extern "C" int pbb_destroy(pbb_engine *e)
+{
+    if (!e) return PBB_OK;
+    hipFree(e->mat);
+    hipFree(e->pos);
+    hipFree(e->end);
+    hipFree(e->dir);
+    hipFree(e->line);
+    hipFree(e->state);
+    hipFree(e->ring_id);
+    hipFree(e->ring_seq);
+    hipFree(e->ctl);
+    hipFree(e->best);
+    hipFree(e->best_perm);
+    hipFree(e->root_row);
+    hipFree(e->root_dir);
+    hipFree(e->root_perm_d);
+    if (e->host_flag) hipHostFree(e->host_flag);
+    if (e->ev0) hipEventDestroy(e->ev0);
+    if (e->ev1) hipEventDestroy(e->ev1);
+    if (e->stream) hipStreamDestroy(e->stream);
+    free_tables(e->D);
+    delete e;
+    return PBB_OK;
+}
+
+extern "C" int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cfg, int ub,
+                          const int *ub_perm, pbb_engine **out)
+{
+    if (!out || !cfg) return fail(PBB_E_ARG, "null argument");
+    *out = nullptr;
+    int rc = check_shape(ptm, n, m);
+    if (rc) return rc;
+    if ((rc = validate(cfg))) return rc;
+    if ((rc = check_device())) return rc;
+    pbb_engine *e = new pbb_engine();
+    e->cfg = *cfg;
+    e->n = n;
+    e->m = m;
+    e->inc_cost = ub;
+    e->initial_ub = ub;
+    e->inc_perm.resize(n);
+    for (int i = 0; i < n; i++) e->inc_perm[i] = ub_perm ? ub_perm[i] : i;
+    e->root_perm.resize(n);
+    for (int i = 0; i < n; i++) e->root_perm[i] = i;
+    build_tables(ptm, n, m, e->T);
+#define TRY(x)                \
+    do {                      \
+        int r_ = (x);         \
+        if (r_) {             \
+            pbb_destroy(e);   \
+            return r_;        \
+        }                     \
+    } while (0)
+#define HTRY(x) TRY(([&]() -> int { HIPCHK(x); return PBB_OK; })())
+    TRY(upload_tables(e->T, e->D));
+    HTRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HTRY(hipEventCreate(&e->ev0));
+    HTRY(hipEventCreate(&e->ev1));
+    int cap = 0;
+    TRY(capacity(e, &cap));
+    e->E = std::max(1, std::min(cfg->nb_ivms, cap));
+    e->blocks = (e->E + 3) / 4;
+    e->ring_cap = 1;
+    while (e->ring_cap < 2 * e->E) e->ring_cap <<= 1;
+    size_t E = (size_t)e->E;
+    HTRY(hipMalloc(&e->mat, E * n * 64));
+    HTRY(hipMalloc(&e->pos, E * 64));
+    HTRY(hipMalloc(&e->end, E * 64));
+    HTRY(hipMalloc(&e->dir, E * 64));
+    HTRY(hipMalloc(&e->line, E * 4));
+    HTRY(hipMalloc(&e->state, E * 4));
+    HTRY(hipMalloc(&e->ring_id, (size_t)e->ring_cap * 4));
+    HTRY(hipMalloc(&e->ring_seq, (size_t)e->ring_cap * 4));
+    HTRY(hipMalloc(&e->ctl, sizeof(Ctl)));
+    HTRY(hipMalloc(&e->best, 8));
+    HTRY(hipMalloc(&e->best_perm, E * 64 * 4));
+    HTRY(hipMalloc(&e->root_row, 64 * 4));
+    HTRY(hipMalloc(&e->root_dir, 4));
+    HTRY(hipMalloc(&e->root_perm_d, 64 * 4));
+    HTRY(hipHostMalloc((void **)&e->host_flag, 4, hipHostMallocMapped));
+    *e->host_flag = 0;
+    HTRY(hipHostGetDevicePointer((void **)&e->host_flag_d, e->host_flag, 0));
+    int dev = 0;
+    HTRY(hipGetDevice(&dev));
+    HTRY(hipDeviceGetAttribute(&e->wall_khz, hipDeviceAttributeWallClockRate, dev));
+    HTRY(hipMemset(e->state, 0, E * 4));
+    unsigned long long packed = ((unsigned long long)(unsigned)ub << 32) | 0xffffffffull;
+    HTRY(hipMemcpy(e->best, &packed, 8, hipMemcpyHostToDevice));
+#undef HTRY
+#undef TRY
+    *out = e;
+    return PBB_OK;
+}
+
+extern "C" int pbb_set_root(pbb_engine *e, const int *root_perm)
+{
+    if (!e || !root_perm) return fail(PBB_E_ARG, "null argument");
+    std::vector<int> seen(e->n, 0);
+    for (int i = 0; i < e->n; i++) {
+        if (root_perm[i] < 0 || root_perm[i] >= e->n || seen[root_perm[i]]++)
+            return fail(PBB_E_ARG, "root order is not a permutation");
+    }
+    e->root_perm.assign(root_perm, root_perm + e->n);
+    e->root_bounded = false;
+    return PBB_OK;
+}
+
+static int sync_incumbent(pbb_engine *e)
+{
+    unsigned long long packed = 0;
+    HIPCHK(hipMemcpy(&packed, e->best, 8, hipMemcpyDeviceToHost));
+    int cost = (int)(packed >> 32);
+    unsigned who = (unsigned)(packed & 0xffffffffull);
+    if (cost < e->inc_cost && who != 0xffffffffu && (int)who < e->E) {
+        std::vector<int> p(64);
+        HIPCHK(hipMemcpy(p.data(), e->best_perm + (size_t)who * 64, 64 * 4, hipMemcpyDeviceToHost));
+        e->inc_cost = cost;
+        e->inc_perm.assign(p.begin(), p.begin() + e->n);
+    }
+    return PBB_OK;
+}
+
+static int bound_root(pbb_engine *e)
+{
+    if (e->root_bounded) return PBB_OK;
+    int rc = sync_incumbent(e);
+    if (rc) return rc;
+    e->initial_ub = e->inc_cost;
+    std::vector<int> rp(64, 0);
+    std::copy(e->root_perm.begin(), e->root_perm.end(), rp.begin());
+    HIPCHK(hipMemcpy(e->root_perm_d, rp.data(), 64 * 4, hipMemcpyHostToDevice));
+    KArgs a = make_args(e);
+    HIPCHK(launch_root(a, e->root_perm_d, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->root_bounded = true;
+    return PBB_OK;
+}
+
+// reset the explorer pool: explorers in `active` get (pos, end) and unfold,
+// the others are registered idle.
+static int setup_pool(pbb_engine *e, const std::vector<int> &ids, const int *pos, const int *end)
+{
+    const int n = e->n, E = e->E;
+    std::vector<int> state(E, ST_IDLE);
+    std::vector<int8_t> hp((size_t)E * 64, 0), he((size_t)E * 64, 0);
+    for (size_t k = 0; k < ids.size(); k++) {
+        int id = ids[k];
+        state[id] = ST_UNFOLD;
+        for (int i = 0; i < n; i++) {
+            hp[(size_t)id * 64 + i] = (int8_t)pos[k * n + i];
+            he[(size_t)id * 64 + i] = (int8_t)end[k * n + i];
+        }
+    }
+    std::vector<int> ring_id(e->ring_cap, 0);
+    std::vector<unsigned> ring_seq(e->ring_cap);
+    for (int i = 0; i < e->ring_cap; i++) ring_seq[i] = (unsigned)i;
+    unsigned tail = 0;
+    for (int i = 0; i < E; i++)
+        if (state[i] == ST_IDLE) {
+            ring_id[tail] = i + 1;
+            ring_seq[tail] = tail + 1;
+            tail++;
+        }
+    Ctl c;
+    memset(&c, 0, sizeof c);
+    c.idle_head = 0;
+    c.idle_tail = tail;
+    c.work_count = (int)ids.size();
+    c.done = ids.empty() ? 1 : 0;
+    HIPCHK(hipMemcpy(e->state, state.data(), (size_t)E * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->pos, hp.data(), hp.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->end, he.data(), he.size(), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ring_id, ring_id.data(), (size_t)e->ring_cap * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ring_seq, ring_seq.data(), (size_t)e->ring_cap * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->ctl, &c, sizeof c, hipMemcpyHostToDevice));
+    e->exported_dec = e->exported_leaves = 0;
+    e->initialized = true;
+    return PBB_OK;
+}
+
+extern "C" int pbb_init_full(pbb_engine *e)
+{
+    if (!e) return fail(PBB_E_ARG, "null engine");
+    int rc = bound_root(e);
+    if (rc) return rc;
+    std::vector<int> p(e->n, 0), q(e->n);
+    for (int i = 0; i < e->n; i++) q[i] = e->n - i - 1;
+    return setup_pool(e, std::vector<int>{0}, p.data(), q.data());
+}
+
+extern "C" int pbb_init_intervals(pbb_engine *e, int nb, const int *ids, const int *pos, const int *end)
+{
+    if (!e || nb < 0 || (nb > 0 && (!ids || !pos || !end))) return fail(PBB_E_ARG, "bad arguments");
+    if (nb > e->E) return fail(PBB_E_CAPACITY, "cannot handle more than %d intervals", e->E);
+    std::vector<int> seen(e->E, 0), idv(nb);
+    for (int k = 0; k < nb; k++) {
+        if (ids[k] < 0 || ids[k] >= e->E || seen[ids[k]]++)
+            return fail(PBB_E_ARG, "interval id %d invalid or repeated", ids[k]);
+        for (int i = 0; i < e->n; i++) {
+            int lim = e->n - i;
+            if (pos[k * e->n + i] < 0 || pos[k * e->n + i] >= lim || end[k * e->n + i] < 0 ||
+                end[k * e->n + i] >= lim)
+                return fail(PBB_E_ARG, "interval %d: factoradic digit %d out of range", k, i);
+        }
+        idv[k] = ids[k];
+    }
+    int rc = bound_root(e);
+    if (rc) return rc;
+    return setup_pool(e, idv, pos, end);
+}
+
+extern "C" int pbb_run(pbb_engine *e, uint64_t node_budget, double time_budget_s, int *all_done)
+{
+    if (!e) return fail(PBB_E_ARG, "null engine");
+    if (!e->initialized) return fail(PBB_E_STATE, "pbb_run before pbb_init_full/pbb_init_intervals");
+    Ctl c;
+    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
+    if (c.done) {
+        if (all_done) *all_done = 1;
+        return PBB_OK;
+    }
+    c.stop = 0;
+    HIPCHK(hipMemcpy(&e->ctl->stop, &c.stop, sizeof(int), hipMemcpyHostToDevice));
+    *e->host_flag = 0;
+    KArgs a = make_args(e);
+    a.node_budget = node_budget ? c.dec + node_budget : 0;
+    a.deadline = time_budget_s > 0 ? (unsigned long long)(time_budget_s * e->wall_khz * 1000.0) : 0;
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    HIPCHK(launch_explore(a, e->blocks, e->stream));
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+    e->last_ms = ms;
+    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
+    if (c.err) return fail(PBB_E_HIP, "device error %d in the explorer kernel", c.err);
+    int rc = sync_incumbent(e);
+    if (rc) return rc;
+    if (all_done) *all_done = c.done ? 1 : 0;
+    return PBB_OK;
+}
+
+extern "C" int pbb_get_intervals(pbb_engine *e, int *pos, int *end, int *ids, int *nb, int max,
+                                 uint64_t *decomposed, uint64_t *leaves)
+{
+    if (!e || !pos || !end || !ids || !nb) return fail(PBB_E_ARG, "null argument");
+    if (max < e->E) return fail(PBB_E_CAPACITY, "buffer too small (%d < %d)", max, e->E);
+    const int n = e->n, E = e->E;
+    std::vector<int> st(E);
+    std::vector<int8_t> hp((size_t)E * 64), he((size_t)E * 64);
+    HIPCHK(hipMemcpy(st.data(), e->state, (size_t)E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(hp.data(), e->pos, hp.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(he.data(), e->end, he.size(), hipMemcpyDeviceToHost));
+    memset(pos, 0, sizeof(int) * (size_t)max * n);
+    memset(end, 0, sizeof(int) * (size_t)max * n);
+    int k = 0;
+    for (int i = 0; i < E; i++)
+        if (st[i] != ST_IDLE) {
+            ids[k] = i;
+            for (int d = 0; d < n; d++) {
+                pos[(size_t)k * n + d] = hp[(size_t)i * 64 + d];
+                end[(size_t)k * n + d] = he[(size_t)i * 64 + d];
+            }
+            k++;
+        }
+    *nb = k;
+    Ctl c;
+    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
+    if (decomposed) *decomposed = c.dec - e->exported_dec;
+    if (leaves) *leaves = c.leaves - e->exported_leaves;
+    e->exported_dec = c.dec;
+    e->exported_leaves = c.leaves;
+    return PBB_OK;
+}
+
+extern "C" int pbb_get_incumbent(pbb_engine *e, int *cost, int *perm)
+{
+    if (!e || !cost) return fail(PBB_E_ARG, "null argument");
+    int rc = sync_incumbent(e);
+    if (rc) return rc;
+    *cost = e->inc_cost;
+    if (perm) std::copy(e->inc_perm.begin(), e->inc_perm.end(), perm);
+    return PBB_OK;
+}
+
+extern "C" int pbb_set_incumbent(pbb_engine *e, int cost, const int *perm)
+{
+    if (!e) return fail(PBB_E_ARG, "null engine");
+    int rc = sync_incumbent(e);
+    if (rc) return rc;
+    if (cost >= e->inc_cost) return PBB_OK;  // solution::update keeps strict improvements
+    e->inc_cost = cost;
+    if (perm) e->inc_perm.assign(perm, perm + e->n);
+    unsigned long long packed = ((unsigned long long)(unsigned)cost << 32) | 0xffffffffull;
+    HIPCHK(hipMemcpy(e->best, &packed, 8, hipMemcpyHostToDevice));
+    return PBB_OK;
+}
+
+extern "C" int pbb_get_stats(pbb_engine *e, pbb_stats *st)
+{
+    if (!e || !st) return fail(PBB_E_ARG, "null argument");
+    Ctl c;
+    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
+    memset(st, 0, sizeof *st);
+    st->decomposed = c.dec;
+    st->leaves = c.leaves;
+    st->simple_bounds = c.sb;
+    st->johnson_bounds = c.jb;
+    st->explorers = e->E;
+    std::vector<int> s(e->E);
+    HIPCHK(hipMemcpy(s.data(), e->state, (size_t)e->E * 4, hipMemcpyDeviceToHost));
+    int act = 0;
+    for (int v : s) act += v != ST_IDLE;
+    st->active = act;
+    st->kernel_ms = e->last_ms;
+    for (int i = 0; i <= PBB_MAX_JOBS + 1; i++) st->depth_histo[i] = c.histo[i];
+    return PBB_OK;
+}
+
+extern "C" int pbb_interrupt(pbb_engine *e)
+{
+    if (!e) return fail(PBB_E_ARG, "null engine");
+    __atomic_store_n(e->host_flag, 1, __ATOMIC_SEQ_CST);
+    return PBB_OK;
+}
+
+extern "C" int pbb_query_explorers(int n, int m, const pbb_config *cfg, int *explorers)
+{
+    if (!cfg || !explorers) return fail(PBB_E_ARG, "null argument");
+    std::vector<int32_t> ptm((size_t)n * m, 1);
+    pbb_engine *e = nullptr;
+    int rc = pbb_create(ptm.data(), n, m, cfg, INT_MAX, nullptr, &e);
+    if (rc) return rc;
+    *explorers = e->E;
+    return pbb_destroy(e);
+}

@@ -1,0 +1,1069 @@
+// pbb_kernels.hip -- gfx950 kernels of the IVM branch-and-bound engine.
+//
+//   k_explore<M,REF> persistent explorer kernel: one IVM per 64-lane
+//                  wavefront, 4 explorers per 256-thread workgroup, the IVM
+//                  matrix of each explorer staged in LDS (int8, 64-byte rows),
+//                  the interval vectors and the front/back stacks in VGPRs.
+//                  Implements sequentialbb::next / unfold
+//                  (multicore/src/sequentialbb.cpp:110-264) with ivm.cpp
+//                  navigation and ivm_bound.cpp bounding/branching/pruning.
+//                  Work is balanced on the device: an explorer whose interval
+//                  is exhausted registers in an idle ring; busy explorers
+//                  split their interval as bbthread::shareWork
+//                  (multicore/src/bbthread.cpp:39-72) and hand the upper part
+//                  to an idle explorer.  No host round trip until the run
+//                  ends (all idle, node budget, deadline or interrupt).
+//                  REF=false compiles the LB1-only path (boundingMode 0).
+//   k_bound_root<M> LB1 root bounding, ivm_bound::boundRoot (ivm_bound.cpp:273-330).
+//   k_bounds<M>    batched bound_abstract calls over device buffers, one
+//                  partial schedule per wavefront, on the explorer's own
+//                  routines (LB1 children, LB1', LB2, makespan).
+//
+// Translation units: this file is compiled once per machine count M
+// (-DPBB_M=<m>) so the 19 instantiations build in parallel.
+#include "pbb_device.h"
+#include "pbb_kernels.h"
+
+namespace pbb {
+
+// ------------------------------------------------------------------ atomics
+__device__ __forceinline__ int ld_acq(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_rlx(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_rlx(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// lane-0 read-modify-writes, result broadcast to the wavefront
+__device__ __forceinline__ int w_add(int *p, int v)
+{
+    int r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return rdlane(r, 0);
+}
+__device__ __forceinline__ unsigned w_addu(unsigned *p, unsigned v)
+{
+    unsigned r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return (unsigned)rdlane((int)r, 0);
+}
+__device__ __forceinline__ unsigned long long w_add64(unsigned long long *p, unsigned long long v)
+{
+    unsigned long long r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned lo = (unsigned)rdlane((int)(unsigned)r, 0);
+    unsigned hi = (unsigned)rdlane((int)(unsigned)(r >> 32), 0);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ bool w_cas(unsigned *p, unsigned expect, unsigned desired)
+{
+    int ok = 0;
+    if (lane_id() == 0) {
+        unsigned e = expect;
+        ok = __hip_atomic_compare_exchange_strong(p, &e, desired, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return rdlane(ok, 0) != 0;
+}
+__device__ __forceinline__ void w_min64(unsigned long long *p, unsigned long long v)
+{
+    if (lane_id() == 0) __hip_atomic_fetch_min(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------ LDS layout
+// [ptmT n*mp][min_arr mp][min_dep mp][tot mp][jtab npairs*n int4 (LB2 only)]
+// then per wave: [matrix n*64 bytes][scratch 64 ints]
+struct Lds {
+    int *ptmT;
+    int *minA, *minD, *tot;
+    int4 *jtab;
+    int8_t *mat;   // this wave's matrix, row r at mat + r*64
+    int *scr;      // this wave's scratch
+};
+
+__device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load_jtab)
+{
+    Lds L;
+    const int n = I.n, mp = I.mp;
+    L.ptmT = (int *)smem;
+    L.minA = L.ptmT + n * mp;
+    L.minD = L.minA + mp;
+    L.tot = L.minD + mp;
+    size_t off = (size_t)(n + 3) * mp * 4;
+    off = (off + 15) & ~(size_t)15;
+    load_jtab = load_jtab && jtab_in_lds(n, I.npairs);
+    L.jtab = load_jtab ? (int4 *)(smem + off) : const_cast<int4 *>(I.jtab);
+    char *wbase = smem + lds_table_bytes(n, mp, I.npairs, load_jtab) +
+                  (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n);
+    L.mat = (int8_t *)wbase;
+    L.scr = (int *)(wbase + (size_t)n * 64);
+    for (int i = threadIdx.x; i < n * mp; i += blockDim.x) L.ptmT[i] = I.ptmT[i];
+    for (int i = threadIdx.x; i < mp; i += blockDim.x) {
+        L.minA[i] = i < I.m ? I.min_arr[i] : 0;
+        L.minD[i] = i < I.m ? I.min_dep[i] : 0;
+        int s = 0;
+        if (i < I.m)
+            for (int j = 0; j < n; j++) s += I.ptmT[j * mp + i];
+        L.tot[i] = s;
+    }
+    if (load_jtab)
+        for (int i = threadIdx.x; i < I.npairs * n; i += blockDim.x) L.jtab[i] = I.jtab[i];
+    __syncthreads();
+    return L;
+}
+
+// ------------------------------------------------------------- explorer
+// Register state of one explorer.  Stacks: F[k] lane s = completion time on
+// machine k of the first s front jobs, F[k] lane 0 = min_arr[k] (the front of
+// an empty prefix, bound_fsp_weak.cpp:67-70); B likewise with min_dep.  The
+// chains started from min_arr/min_dep equal those started from zero, since
+// min_arr[k] never exceeds a single job's head (and min_dep its tail).
+template <int M, bool REF>
+struct Explorer {
+    const KArgs &a;
+    Lds lds;
+    int n, mp, lane, e;
+    int L;                            // current line
+    int rowv, posv, endv, dirv, remv; // lane vectors (remv: lane k = remain on machine k)
+    int F[M], B[M];
+    int nf, nb;
+    int local_best;
+    unsigned long long dec, dec_flushed, leaves, sb, jb;
+    int hist;
+    unsigned long long t_end;
+
+    __device__ __forceinline__ Explorer(const KArgs &args, const Lds &l, int eid)
+        : a(args), lds(l), n(args.I.n), mp(args.I.mp), lane(lane_id()), e(eid)
+    {
+        L = 0;
+        rowv = posv = endv = dirv = remv = 0;
+        nf = nb = 0;
+        dec = dec_flushed = leaves = sb = jb = 0;
+        hist = 0;
+        local_best = PBB_INF;
+        t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
+#pragma unroll
+        for (int k = 0; k < M; k++) F[k] = B[k] = 0;
+    }
+
+    __device__ __forceinline__ bool gt_best(int v) const
+    {
+        return a.find_all ? v > local_best : v >= local_best;
+    }
+
+    __device__ __forceinline__ int pmach(int job) const  // lane k: p[k][job]
+    {
+        return lane < M ? lds.ptmT[job * mp + lane] : 0;
+    }
+
+    __device__ __forceinline__ void reset_stacks()
+    {
+        nf = nb = 0;
+        int ma = lane < M ? lds.minA[lane] : 0;
+        int md = lane < M ? lds.minD[lane] : 0;
+#pragma unroll
+        for (int k = 0; k < M; k++) {
+            F[k] = wrlane(0, rdlane(ma, k), 0);
+            B[k] = wrlane(0, rdlane(md, k), 0);
+        }
+        remv = lane < M ? lds.tot[lane] : 0;
+    }
+
+    // schedule job j in direction d (push on F or B); remain -= p_j
+    __device__ __forceinline__ void push_job(int j, int d)
+    {
+        int pk = pmach(j);
+        if (d == 0) {
+            int c = 0;
+#pragma unroll
+            for (int k = 0; k < M; k++) {
+                int base = rdlane(F[k], nf);
+                c = (k == 0 ? base : max(base, c)) + rdlane(pk, k);
+                F[k] = wrlane(F[k], c, nf + 1);
+            }
+            nf++;
+        } else {
+            int c = 0;
+#pragma unroll
+            for (int k = M - 1; k >= 0; k--) {
+                int base = rdlane(B[k], nb);
+                c = (k == M - 1 ? base : max(base, c)) + rdlane(pk, k);
+                B[k] = wrlane(B[k], c, nb + 1);
+            }
+            nb++;
+        }
+        remv -= pk;
+    }
+
+    __device__ __forceinline__ void pop_job(int j, int d)
+    {
+        if (d == 0)
+            nf--;
+        else
+            nb--;
+        remv += pmach(j);
+    }
+
+    // ivm::beforeEnd (ivm.cpp:152-160): lexicographic pos <= end
+    __device__ __forceinline__ bool before_end() const
+    {
+        bool in = lane < n;
+        uint64_t lt = ballot(in && posv < endv);
+        uint64_t gt = ballot(in && posv > endv);
+        uint64_t d = lt | gt;
+        if (d == 0) return true;
+        int f = __ffsll((unsigned long long)d) - 1;
+        return (lt >> f) & 1ull;
+    }
+
+    // ------------------------------------------------------- LB1 children
+    // bound_fsp_weak::addFrontAndBound / addBackAndBound (bound_fsp_weak.cpp:133-191)
+    // for the job of this lane; the uniform front/back/remain values are
+    // read from the stacks one machine at a time.
+    __device__ __forceinline__ int lb1_front(const int *pj, int *chF) const
+    {
+        int f0 = rdlane(F[0], nf);
+        int t0 = f0 + pj[0];
+        int lb = f0 + rdlane(remv, 0) + rdlane(B[0], nb);
+        if (chF) chF[0] = t0;
+#pragma unroll
+        for (int k = 1; k < M; k++) {
+            int fk = rdlane(F[k], nf);
+            int rbk = rdlane(remv, k) + rdlane(B[k], nb);
+            int t1 = max(fk, t0);
+            lb = max(lb, t1 + rbk);
+            t0 = t1 + pj[k];
+            if (chF) chF[k] = t0;
+        }
+        return lb;
+    }
+
+    __device__ __forceinline__ int lb1_back(const int *pj, int *chB) const
+    {
+        int bl = rdlane(B[M - 1], nb);
+        int t0 = bl + pj[M - 1];
+        int lb = rdlane(F[M - 1], nf) + rdlane(remv, M - 1) + bl;
+        if (chB) chB[M - 1] = t0;
+#pragma unroll
+        for (int k = M - 2; k >= 0; k--) {
+            int bk = rdlane(B[k], nb);
+            int frk = rdlane(F[k], nf) + rdlane(remv, k);
+            int t1 = max(t0, bk);
+            lb = max(lb, frk + t1);
+            t0 = t1 + pj[k];
+            if (chB) chB[k] = t0;
+        }
+        return lb;
+    }
+
+    // chooseChildrenSet (ivm_bound.cpp:413-507): lanes beyond the row hold
+    // the reference's zero entries and are excluded from every term.
+    __device__ __forceinline__ int choose(bool valid, int cb, int ce) const
+    {
+        switch (a.branching) {
+        case -3: return (L % 2) ? 1 : 0;
+        case -2: return 1;
+        case -1: return 0;
+        case 1: {
+            int s = wave_sum(valid ? cb - ce : 0);
+            return s < 0 ? 1 : 0;
+        }
+        case 2: {
+            int ub = a.single_node ? local_best : a.initial_ub;
+            int elim = __popcll(ballot(valid && cb >= ub)) - __popcll(ballot(valid && ce >= ub));
+            int s = wave_sum(valid ? ((cb < ub ? cb : 0) - (ce < ub ? ce : 0)) : 0);
+            if (elim > 0) return 0;
+            if (elim < 0) return 1;
+            return s < 0 ? 1 : 0;
+        }
+        default: {
+            int ub = a.single_node ? local_best : a.initial_ub;
+            int mn = wave_min(valid ? min(cb > 0 ? cb : PBB_INF, ce > 0 ? ce : PBB_INF) : PBB_INF);
+            int minc = __popcll(ballot(valid && cb == mn)) - __popcll(ballot(valid && ce == mn));
+            int elimc = __popcll(ballot(valid && cb >= ub)) - __popcll(ballot(valid && ce >= ub));
+            if (minc > 0) return 1;
+            if (minc < 0) return 0;
+            return elimc > 0 ? 0 : 1;
+        }
+        }
+    }
+
+    // ----------------------------------------------------------- bounding
+    // Bound the children of the node at line L (row in rowv, unflagged),
+    // choose the direction, refine, prune, reverse (ivm_bound.cpp:141-236,
+    // :337-676).  root: boundRoot always uses weakBoundPrune.
+    __device__ __forceinline__ void bound_row(bool root)
+    {
+        const int k = n - L;
+        const bool valid = lane < k;
+        const int job = valid ? rowv : 0;
+        const bool both = a.branching > 0;
+        const int *pjp = lds.ptmT + job * mp;
+        int pj[M];
+#pragma unroll
+        for (int q = 0; q < M; q++) pj[q] = pjp[q];
+
+        int dir;
+        int c1, c2 = 0;
+        bool refined = false;
+        if constexpr (!REF) {
+            int lbF = lb1_front(pj, nullptr);
+            int lbB = both ? lb1_back(pj, nullptr) : 0;
+            sb += (unsigned long long)(both ? 2 * k : k);
+            dir = choose(valid, lbF, lbB);
+            c1 = dir ? lbB : lbF;
+        } else {
+            const int mode = root ? 0 : a.bound_mode;
+            int chF[M], chB[M];
+            int lbF = lb1_front(pj, chF);
+            int lbB = (both || mode == 1) ? lb1_back(pj, chB) : 0;
+            if (mode == 1) {
+                int csF = 0, csB = 0;
+                refine_both(valid, job, pj, chF, chB, csF, csB);
+                dir = choose(valid, csF, csB);
+                c1 = c2 = dir ? csB : csF;
+                refined = true;
+            } else {
+                if (!both) lbB = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
+                sb += (unsigned long long)(both ? 2 * k : k);
+                dir = choose(valid, lbF, lbB);
+                c1 = dir ? lbB : lbF;
+                if (mode == 2) {
+                    c2 = refine_dir(dir, valid, job, pj, chF, chB, c1);
+                    refined = true;
+                }
+            }
+        }
+        bool pr = valid && (gt_best(c1) || (refined && gt_best(c2)));
+        int v = pr ? flag_neg(job) : job;
+        int prev = L > 0 ? rdlane(dirv, L - 1) : 0;
+        if (prev != dir) v = shfl(v, k - 1 - lane);
+        rowv = valid ? v : 0;
+        dirv = wrlane(dirv, dir, L);
+    }
+
+    // ---------------------------------------------------------- refinement
+    __device__ __forceinline__ uint64_t row_jobs(bool valid, int job) const
+    {
+        uint64_t bits = valid ? (1ull << job) : 0ull;
+        unsigned lo = (unsigned)bits, hi = (unsigned)(bits >> 32);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo |= (unsigned)__shfl_xor((int)lo, o, PBB_WAVE);
+            hi |= (unsigned)__shfl_xor((int)hi, o, PBB_WAVE);
+        }
+        return ((uint64_t)(unsigned)uni((int)hi) << 32) | (unsigned)uni((int)lo);
+    }
+
+    // LB1' of this lane's child (bound_fsp_weak::bornes_calculer with the
+    // child's front or back): d=0 front' = chF, d=1 back' = chB.
+    __device__ __forceinline__ int lb1p_child(int d, const int *pj, const int *chF, const int *chB) const
+    {
+        int t0 = 0, lb = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int f = d == 0 ? chF[q] : rdlane(F[q], nf);
+            int b = d == 1 ? chB[q] : rdlane(B[q], nb);
+            int r = rdlane(remv, q) - pj[q];
+            if (q == 0) {
+                t0 = f + r;
+                lb = t0 + b;
+            } else {
+                int t1 = max(f + r, t0);
+                lb = max(lb, t1 + b);
+                t0 = t1;
+            }
+        }
+        return lb;
+    }
+
+    // Johnson LB2 of child c (uniform), direction d
+    // (bound_fsp_strong::bornes_calculer with the child's partial schedule)
+    __device__ __forceinline__ int lb2_child(int c, int d, const int *chF, const int *chB,
+                                             uint64_t rowmask, int k)
+    {
+        const int cj = flag_abs(rdlane(rowv, c));
+        if (k <= 2) {
+            // l2 - l1 <= 2: makespan of the completed permutation (bound_fsp_strong.cpp:543-546)
+            const int other = rdlane(rowv, 1 - c);
+            int fr[M];
+#pragma unroll
+            for (int q = 0; q < M; q++) fr[q] = d == 0 ? rdlane(chF[q], c) : rdlane(F[q], nf);
+            int po = pmach(flag_abs(other));
+            int cc = 0, best = 0;
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+                cc = (q == 0 ? fr[0] : max(fr[q], cc)) + rdlane(po, q);
+                int bk = d == 1 ? rdlane(chB[q], c) : rdlane(B[q], nb);
+                best = max(best, cc + bk);
+            }
+            return best;
+        }
+        int *fs = lds.scr, *bs = lds.scr + 32;
+        int fl = 0, bl = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int fq = d == 0 ? rdlane(chF[q], c) : rdlane(F[q], nf);
+            int bq = d == 1 ? rdlane(chB[q], c) : rdlane(B[q], nb);
+            if (lane == q) {
+                fl = fq;
+                bl = bq;
+            }
+        }
+        if (lane < M) {
+            fs[lane] = fl;
+            bs[lane] = bl;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint64_t unsched = rowmask & ~(1ull << cj);
+        DevInst J = a.I;
+        J.jtab = lds.jtab;
+        int r = lb2_pairs_wave(J, fs, bs, unsched, a.pairs_mode);
+        __builtin_amdgcn_wave_barrier();
+        return r;
+    }
+
+    // mixed mode (computeStrongBounds, ivm_bound.cpp:141-195): refine the
+    // children not pruned by LB1 in the chosen direction; 0 for the others
+    __device__ __forceinline__ int refine_dir(int d, bool valid, int job, const int *pj, const int *chF,
+                                              const int *chB, int cw)
+    {
+        const int k = n - L;
+        bool todo = valid && !gt_best(cw);
+        uint64_t tmask = ballot(todo);
+        jb += __popcll(tmask);
+        if (a.refine == 0) {
+            int v = lb1p_child(d, pj, chF, chB);
+            return todo ? v : 0;
+        }
+        uint64_t rowmask = row_jobs(valid, job);
+        int cs = 0;
+        uint64_t m = tmask;
+        while (m) {
+            int c = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            int v = lb2_child(c, d, chF, chB, rowmask, k);
+            if (lane == c) cs = v;
+        }
+        return cs;
+    }
+
+    // strong-only mode (strongBoundPrune, ivm_bound.cpp:668-676)
+    __device__ __forceinline__ void refine_both(bool valid, int job, const int *pj, const int *chF,
+                                                const int *chB, int &csF, int &csB)
+    {
+        const int k = n - L;
+        jb += 2ull * k;
+        if (a.refine == 0) {
+            csF = valid ? lb1p_child(0, pj, chF, chB) : 0;
+            csB = valid ? lb1p_child(1, pj, chF, chB) : 0;
+            return;
+        }
+        uint64_t rowmask = row_jobs(valid, job);
+        csF = csB = 0;
+        for (int c = 0; c < k; c++) {
+            int v0 = lb2_child(c, 0, chF, chB, rowmask, k);
+            int v1 = lb2_child(c, 1, chF, chB, rowmask, k);
+            if (lane == c) {
+                csF = v0;
+                csB = v1;
+            }
+        }
+    }
+
+    // -------------------------------------------------------------- leaf
+    // ivm_bound::boundLeaf (ivm_bound.cpp:238-271): makespan of front + the
+    // last row job + back; the incumbent only moves on strict improvement.
+    __device__ __forceinline__ void leaf()
+    {
+        leaves++;
+        const int r = rdlane(rowv, 0);
+        const int pr = pmach(r);
+        int c = 0, cost = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int base = rdlane(F[q], nf);
+            c = (q == 0 ? base : max(base, c)) + rdlane(pr, q);
+            cost = max(cost, c + rdlane(B[q], nb));
+        }
+        if (!gt_best(cost)) {
+            if (cost < local_best) {
+                write_perm(r);
+                w_min64(a.best, ((unsigned long long)(unsigned)cost << 32) | (unsigned)e);
+            }
+            local_best = cost;
+        }
+        rowv = wrlane(rowv, flag_neg(r), 0);
+    }
+
+    // decode the leaf schedule (ivm_bound::prepareSchedule) into best_perm[e]
+    __device__ __forceinline__ void write_perm(int mid)
+    {
+        int jl = 0;
+        if (lane < L) jl = flag_abs((int)lds.mat[lane * 64 + posv]);
+        uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        uint64_t fr = ballot(lane < L && dirv == 0);
+        uint64_t bk = ballot(lane < L && dirv != 0);
+        int *dst = a.best_perm + (size_t)e * 64;
+        if (lane < L) {
+            int p = dirv == 0 ? __popcll(fr & below) : n - 1 - __popcll(bk & below);
+            dst[p] = jl;
+        }
+        if (lane == 0) dst[__popcll(fr)] = mid;
+    }
+
+    // ------------------------------------------------------- navigation
+    __device__ __forceinline__ void store_row(int line) const
+    {
+        if (lane < n - line) lds.mat[line * 64 + lane] = (int8_t)rowv;
+    }
+
+    __device__ __forceinline__ void go_up()
+    {
+        if (L > 0) {
+            posv = wrlane(posv, 0, L);
+            L--;
+            rowv = lane < n - L ? (int)lds.mat[L * 64 + lane] : 0;
+            const int p = rdlane(posv, L);
+            pop_job(flag_abs(rdlane(rowv, p)), rdlane(dirv, L));
+            if (lane == p) rowv = flag_neg(rowv);
+        } else {
+            posv = wrlane(posv, rdlane(posv, 0) + 1, 0);
+        }
+    }
+
+    // branch on cell c of line L: generate line L+1 (ivm::goDown/generateLine)
+    __device__ __forceinline__ void go_down(int c, bool explore)
+    {
+        store_row(L);
+        push_job(flag_abs(rdlane(rowv, c)), rdlane(dirv, L));
+        int v = shfl(rowv, lane + (lane >= c ? 1 : 0));
+        L++;
+        rowv = lane < n - L ? flag_abs(v) : 0;
+        if (explore) {
+            posv = wrlane(posv, 0, L);
+            dirv = wrlane(dirv, 0, L);
+        }
+    }
+
+    // sequentialbb::unfold (sequentialbb.cpp:206-264): bounded, uncounted
+    __device__ __forceinline__ void unfold()
+    {
+        while (L < n - 2) {
+            int p = rdlane(posv, L);
+            if (rdlane(rowv, p) < 0) {
+                if (lane > L) posv = 0;
+                break;
+            }
+            go_down(p, false);
+            bound_row(false);
+        }
+    }
+
+    // One call of sequentialbb::next: returns 1 when an interior node was
+    // decomposed and bounded, 0 when the interval is exhausted.
+    __device__ __forceinline__ int step()
+    {
+        while (true) {
+            if (!before_end()) return 0;
+            const int p = rdlane(posv, L);
+            if (p >= n - L) {
+                go_up();
+                continue;
+            }
+            uint64_t live = ballot(lane < n - L && rowv >= 0);
+            uint64_t ahead = live & (~0ull << p);
+            if (!((ahead >> p) & 1ull)) {
+                // goRight across pruned / visited cells (ivm::goRight)
+                int c = ahead ? __ffsll((unsigned long long)ahead) - 1 : n - L;
+                posv = wrlane(posv, c, L);
+                continue;
+            }
+            dec++;
+            go_down(p, true);
+            hist += (lane == L);
+            if (L == n - 1) {
+                leaf();
+                continue;
+            }
+            bound_row(false);
+            return 1;
+        }
+    }
+
+    // ------------------------------------------------------ state I/O
+    __device__ __forceinline__ void rebuild_stacks()
+    {
+        reset_stacks();
+        for (int l = 0; l < L; l++) {
+            int p = rdlane(posv, l);
+            int j = uni(flag_abs((int)lds.mat[l * 64 + p]));
+            push_job(j, rdlane(dirv, l));
+        }
+    }
+
+    __device__ __forceinline__ void load_state()
+    {
+        L = uni(a.line[e]);
+        const int8_t *gm = a.mat + (size_t)e * n * 64;
+        for (int r = 0; r < L; r++)
+            if (lane < n) lds.mat[r * 64 + lane] = gm[r * 64 + lane];
+        rowv = lane < n - L ? (int)gm[L * 64 + lane] : 0;
+        posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
+        endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
+        dirv = lane < n ? (int)a.dir[(size_t)e * 64 + lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        rebuild_stacks();
+    }
+
+    __device__ __forceinline__ void save_state(int st)
+    {
+        int8_t *gm = a.mat + (size_t)e * n * 64;
+        for (int r = 0; r < L; r++)
+            if (lane < n) gm[r * 64 + lane] = lds.mat[r * 64 + lane];
+        if (lane < n) gm[L * 64 + lane] = (int8_t)rowv;
+        if (lane < n) {
+            a.pos[(size_t)e * 64 + lane] = (int8_t)posv;
+            a.end[(size_t)e * 64 + lane] = (int8_t)endv;
+            a.dir[(size_t)e * 64 + lane] = (int8_t)dirv;
+        }
+        if (lane == 0) a.line[e] = L;
+        if (lane == 0) st_rlx(a.state + e, st);
+    }
+
+    // fresh interval: row 0 = bounded root row, pos/end from HBM
+    __device__ __forceinline__ bool start_unfold()
+    {
+        L = 0;
+        rowv = lane < n ? a.root_row[lane] : 0;
+        posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
+        endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
+        dirv = lane == 0 ? a.root_dir[0] : 0;
+        reset_stacks();
+        if (!before_end()) return false;
+        unfold();
+        return true;
+    }
+
+    // --------------------------------------------------- work sharing
+    __device__ __forceinline__ void fail(int code)
+    {
+        if (lane == 0) {
+            st_rlx(&a.ctl->err, code);
+            st_rlx(&a.ctl->stop, 1);
+        }
+    }
+
+    // bbthread::shareWork + ivm::cuttingPosition (bbthread.cpp:39-72,
+    // ivm.cpp:188-237): split at the first line l whose pos differs from end
+    // (l <= L, l < n - margin); the thief gets the explorable cells after the
+    // victim's kept half, rows 0..l and the victim's end; no re-bounding.
+    __device__ __forceinline__ void try_donate()
+    {
+        unsigned h = (unsigned)uni((int)ld_rlx(&a.ctl->idle_head));
+        unsigned t = (unsigned)uni((int)ld_rlx(&a.ctl->idle_tail));
+        if (h == t) return;
+        int lim = min(L, n - a.steal_margin);
+        if (lim < 0) lim = 0;
+        uint64_t ne = ballot(lane < n && posv != endv);
+        uint64_t limmask = lim >= 64 ? ~0ull : ((1ull << lim) - 1ull);
+        uint64_t cand = ne & limmask;
+        int l = cand ? __ffsll((unsigned long long)cand) - 1 : lim;
+        int pl = rdlane(posv, l), el = rdlane(endv, l);
+        if (!(pl < el)) return;
+        int rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[l * 64 + lane] : -1);
+        uint64_t expl = ballot(lane > pl && lane <= el && rowl >= 0);
+        int nexp = __popcll(expl);
+        if (nexp < 1) return;
+        if (!w_cas(&a.ctl->idle_head, h, h + 1)) return;
+        // claim the idle explorer registered at ticket h (bounded MPMC ring cell)
+        const unsigned slot = h & (unsigned)(a.ring_cap - 1);
+        bool ok = false;
+        for (int spin = 0; spin < PBB_SPIN_LIMIT; spin++) {
+            if ((unsigned)uni(ld_acq((const int *)(a.ring_seq + slot))) == h + 1u) {
+                ok = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!ok) {
+            fail(PBB_DEV_ERR_RING);
+            return;
+        }
+        const int thief = uni(ld_rlx(a.ring_id + slot)) - 1;
+        if (lane == 0) st_rel((int *)(a.ring_seq + slot), (int)(h + (unsigned)a.ring_cap));
+        if (thief < 0 || thief >= a.E) {
+            fail(PBB_DEV_ERR_RING);
+            return;
+        }
+        int keep = nexp / 2;
+        int cut;
+        if (keep == 0) {
+            cut = pl + 1;
+        } else {
+            uint64_t mm = expl;
+            for (int i = 1; i < keep; i++) mm &= mm - 1;
+            cut = __ffsll((unsigned long long)mm);  // index of the keep-th cell + 1
+        }
+        int8_t *gm = a.mat + (size_t)thief * n * 64;
+        for (int r = 0; r < l; r++)
+            if (lane < n) gm[r * 64 + lane] = lds.mat[r * 64 + lane];
+        if (lane < n) gm[l * 64 + lane] = (int8_t)rowl;
+        if (lane < n) {
+            a.pos[(size_t)thief * 64 + lane] = (int8_t)(lane < l ? posv : (lane == l ? cut : 0));
+            a.end[(size_t)thief * 64 + lane] = (int8_t)endv;
+            a.dir[(size_t)thief * 64 + lane] = (int8_t)(lane <= l ? dirv : 0);
+        }
+        if (lane == 0) a.line[thief] = l;
+        // the victim keeps [pos, cut-1] at line l and everything below
+        if (lane == l) endv = cut - 1;
+        if (lane > l && lane < n) endv = n - 1 - lane;
+        w_add(&a.ctl->work_count, 1);
+        if (lane == 0) st_rel(a.state + thief, ST_ACTIVE);
+    }
+
+    __device__ __forceinline__ void go_idle()
+    {
+        if (lane == 0) st_rlx(a.state + e, ST_IDLE);
+        unsigned t = w_addu(&a.ctl->idle_tail, 1u);
+        const unsigned slot = t & (unsigned)(a.ring_cap - 1);
+        bool ok = false;
+        for (int spin = 0; spin < PBB_SPIN_LIMIT; spin++) {
+            if ((unsigned)uni(ld_acq((const int *)(a.ring_seq + slot))) == t) {
+                ok = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!ok) {
+            fail(PBB_DEV_ERR_RING);
+        } else {
+            if (lane == 0) st_rlx(a.ring_id + slot, e + 1);
+            if (lane == 0) st_rel((int *)(a.ring_seq + slot), (int)(t + 1u));
+        }
+        int left = w_add(&a.ctl->work_count, -1);
+        if (left == 1 && lane == 0) st_rel(&a.ctl->done, 1);
+    }
+
+    __device__ __forceinline__ void flush_counters()
+    {
+        if (lane == 0) {
+            if (dec > dec_flushed) atomicAdd(&a.ctl->dec, dec - dec_flushed);
+            if (leaves) atomicAdd(&a.ctl->leaves, leaves);
+            if (sb) atomicAdd(&a.ctl->sb, sb);
+            if (jb) atomicAdd(&a.ctl->jb, jb);
+        }
+        dec_flushed = dec;
+        leaves = sb = jb = 0;
+        if (hist && lane <= n) atomicAdd(&a.ctl->histo[lane], (unsigned long long)hist);
+        hist = 0;
+    }
+
+    // periodic: node budget, deadline, host interrupt, incumbent refresh
+    __device__ __forceinline__ bool check_stop()
+    {
+        if (a.node_budget) {
+            unsigned long long delta = dec - dec_flushed;
+            unsigned long long before = w_add64(&a.ctl->dec, delta);
+            dec_flushed = dec;
+            if (before + delta >= a.node_budget && lane == 0) st_rlx(&a.ctl->stop, 1);
+        }
+        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(&a.ctl->stop, 1);
+        if (a.host_flag && lane == 0) {
+            int hf = __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (hf) st_rlx(&a.ctl->stop, 1);
+        }
+        unsigned long long b = ld_rlx(a.best);
+        int bc = uni((int)(unsigned)(b >> 32));
+        if (bc < local_best) local_best = bc;
+        return uni(ld_rlx(&a.ctl->stop)) != 0;
+    }
+};
+
+// ------------------------------------------------------------ kernels
+template <int M, bool REF>
+__global__ __launch_bounds__(256) void k_explore(KArgs a)
+{
+    extern __shared__ __align__(16) char smem[];
+    Lds lds = lds_setup(a.I, smem, REF && a.refine == 1);
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= a.E) return;
+
+    Explorer<M, REF> X(a, lds, e);
+    X.local_best = uni((int)(unsigned)(ld_rlx(a.best) >> 32));
+    int st = uni(ld_acq(a.state + e));
+    if (st == ST_UNFOLD) {
+        if (X.start_unfold()) {
+            st = ST_ACTIVE;
+        } else {
+            X.go_idle();
+            st = ST_IDLE;
+        }
+    } else if (st == ST_ACTIVE) {
+        X.load_state();
+    }
+
+    unsigned iter = 0;
+    while (true) {
+        if (st == ST_IDLE) {
+            // wait for a donation, global termination or a stop
+            while (true) {
+                int s = uni(ld_acq(a.state + e));
+                if (s == ST_ACTIVE) {
+                    st = ST_ACTIVE;
+                    X.load_state();
+                    break;
+                }
+                if (uni(ld_rlx(&a.ctl->done)) || uni(ld_rlx(&a.ctl->stop)) ||
+                    (X.t_end && wall_clock64() >= X.t_end)) {
+                    X.flush_counters();
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(8);
+            }
+        }
+        if (!X.step()) {
+            X.go_idle();
+            st = ST_IDLE;
+            continue;
+        }
+        if ((++iter & 63u) == 0u && X.check_stop()) {
+            X.save_state(ST_ACTIVE);
+            X.flush_counters();
+            return;
+        }
+        X.try_donate();
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(64) void k_bound_root(KArgs a, const int *root_perm)
+{
+    extern __shared__ __align__(16) char smem[];
+    Lds lds = lds_setup(a.I, smem, false);
+    Explorer<M, false> X(a, lds, 0);
+    X.local_best = a.initial_ub;
+    X.rowv = X.lane < X.n ? root_perm[X.lane] : 0;
+    X.reset_stacks();
+    X.bound_row(true);
+    int d0 = rdlane(X.dirv, 0);
+    if (X.lane < X.n) a.root_row_out[X.lane] = X.rowv;
+    if (X.lane == 0) a.root_dir_out[0] = d0;
+}
+
+// ------------------------------------------------- batched bound_abstract
+// Load an explicit partial schedule (perm, l1, l2) into an explorer's
+// register stacks: F <- perm[0..l1] (scheduleFront), B <- perm[l2..n-1]
+// in reverse (scheduleBack), remain <- the middle.  Then the explorer's own
+// routines produce the bounds.
+template <int M>
+__device__ __forceinline__ void load_partial(Explorer<M, true> &X, const int *perm, int l1, int l2)
+{
+    X.reset_stacks();
+    for (int i = 0; i <= l1; i++) X.push_job(uni(perm[i]), 0);
+    for (int i = X.n - 1; i >= l2; i--) X.push_job(uni(perm[i]), 1);
+}
+
+// makespan of perm with perm[mid] the only job between the stacks
+// (front = perm[0..mid-1], back = perm[mid+1..n-1]); never pushes an n-th
+// stack entry, so n = 64 keeps every stack index inside the wavefront.
+template <int M>
+__device__ __forceinline__ int makespan_mid(Explorer<M, true> &X, const int *perm, int mid)
+{
+    load_partial<M>(X, perm, mid - 1, mid + 1);
+    const int pr = X.pmach(uni(perm[mid]));
+    int c = 0, cost = 0;
+#pragma unroll
+    for (int q = 0; q < M; q++) {
+        int base = rdlane(X.F[q], X.nf);
+        c = (q == 0 ? base : max(base, c)) + rdlane(pr, q);
+        cost = max(cost, c + rdlane(X.B[q], X.nb));
+    }
+    return cost;
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, const int *perms,
+                                                const int *l1s, const int *l2s, int arg, int *o1, int *o2)
+{
+    extern __shared__ __align__(16) char smem[];
+    Lds lds = lds_setup(a.I, smem, which == 2);
+    const int w = threadIdx.x >> 6;
+    const int s = blockIdx.x * 4 + w;
+    if (s >= count) return;
+    Explorer<M, true> X(a, lds, 0);
+    const int n = X.n, lane = X.lane;
+    const int *perm = perms + (size_t)s * n;
+    // inputs must be permutations of 0..n-1 with -1 <= l1 < l2 <= n;
+    // anything else yields -1 outputs instead of out-of-range table reads
+    const int pv = lane < n ? perm[lane] : 0;
+    bool bad = ballot(lane < n && (pv < 0 || pv >= n)) != 0ull;
+    int l1 = -1, l2 = n;
+    if (which != 3) {
+        l1 = uni(l1s[s]);
+        l2 = uni(l2s[s]);
+        bad = bad || l1 < -1 || l2 > n || l1 >= l2;
+    }
+    if (bad) {
+        if (which == 0) {
+            if (lane < n) o1[(size_t)s * n + lane] = -1;
+            if (arg && lane < n) o2[(size_t)s * n + lane] = -1;
+        } else if (lane == 0) {
+            o1[s] = -1;
+        }
+        return;
+    }
+    if (which == 3) {
+        // bound_fsp_weak::evalSolution
+        int v = makespan_mid<M>(X, perm, n - 1);
+        if (lane == 0) o1[s] = v;
+        return;
+    }
+    const int k = l2 - l1 - 1;
+    if (which == 2 && k <= 1) {
+        // bound_fsp_strong.cpp:543-546: at most one unscheduled job -> makespan
+        int v = makespan_mid<M>(X, perm, k == 1 ? l1 + 1 : (l1 >= 0 ? l1 : 0));
+        if (lane == 0) o1[s] = v;
+        return;
+    }
+    load_partial<M>(X, perm, l1, l2);
+    const bool valid = lane < k;
+    const int job = valid ? perm[l1 + 1 + lane] : 0;
+    if (which == 0) {
+        // bound_fsp_weak::boundChildren (bound_fsp_weak.cpp:193-221)
+        int pj[M];
+#pragma unroll
+        for (int q = 0; q < M; q++) pj[q] = lds.ptmT[job * X.mp + q];
+        int lbF = X.lb1_front(pj, nullptr);
+        int lbB = arg ? X.lb1_back(pj, nullptr) : 0;
+        int *inv = lds.scr;
+        if (lane < n) inv[pv] = lane;
+        __builtin_amdgcn_wave_barrier();
+        int ci = lane < n ? inv[lane] - (l1 + 1) : -1;
+        bool child = ci >= 0 && ci < k;
+        int vF = shfl(lbF, child ? ci : 0);
+        int vB = shfl(lbB, child ? ci : 0);
+        if (lane < n) {
+            o1[(size_t)s * n + lane] = child ? vF : 0;
+            if (arg) o2[(size_t)s * n + lane] = child ? vB : 0;
+        }
+    } else if (which == 1) {
+        // bound_fsp_weak::bornes_calculer (bound_fsp_weak.cpp:288-311)
+        int t0 = 0, lb = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int f = rdlane(X.F[q], X.nf), b = rdlane(X.B[q], X.nb), r = rdlane(X.remv, q);
+            if (q == 0) {
+                t0 = f + r;
+                lb = t0 + b;
+            } else {
+                int t1 = max(f + r, t0);
+                lb = max(lb, t1 + b);
+                t0 = t1;
+            }
+        }
+        if (lane == 0) o1[s] = lb;
+    } else {
+        // bound_fsp_strong::bornes_calculer (bound_fsp_strong.cpp:540-560)
+        uint64_t um = X.row_jobs(valid, job);
+        int *fs = lds.scr, *bs = lds.scr + 32;
+        int fl = 0, bl = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int fq = rdlane(X.F[q], X.nf), bq = rdlane(X.B[q], X.nb);
+            if (lane == q) {
+                fl = fq;
+                bl = bq;
+            }
+        }
+        if (lane < M) {
+            fs[lane] = fl;
+            bs[lane] = bl;
+        }
+        __builtin_amdgcn_wave_barrier();
+        DevInst J = a.I;
+        J.jtab = lds.jtab;
+        int v = lb2_pairs_wave(J, fs, bs, um, arg);
+        if (lane == 0) o1[s] = v;
+    }
+}
+
+// ---------------------------------------------------------- host launchers
+template <int M>
+hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s)
+{
+    const bool lb2 = a.bound_mode != 0 && a.refine == 1;
+    const size_t lds = explore_lds_bytes(a.I, lb2);
+    if (a.bound_mode == 0) {
+        hipFuncSetAttribute((const void *)k_explore<M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((k_explore<M, false>), dim3(blocks), dim3(256), lds, s, a);
+    } else {
+        hipFuncSetAttribute((const void *)k_explore<M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((k_explore<M, true>), dim3(blocks), dim3(256), lds, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
+{
+    const bool lb2 = a.bound_mode != 0 && a.refine == 1;
+    const size_t lds = explore_lds_bytes(a.I, lb2);
+    if (lds > 160 * 1024) {
+        *blocks_per_cu = 0;
+        return hipSuccess;
+    }
+    if (a.bound_mode == 0) {
+        hipFuncSetAttribute((const void *)k_explore<M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, false>, 256, lds);
+    }
+    hipFuncSetAttribute((const void *)k_explore<M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, true>, 256, lds);
+}
+
+template <int M>
+hipError_t launch_root_m(const KArgs &a, const int *root_perm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bound_root<M>, dim3(1), dim3(64), root_lds_bytes(a.I), s, a, root_perm);
+    return hipGetLastError();
+}
+
+template <int M>
+hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perms, const int *l1,
+                           const int *l2, int arg, int *o1, int *o2, hipStream_t s)
+{
+    const bool lb2 = which == 2 && jtab_in_lds(a.I.n, a.I.npairs);
+    const size_t lds = lds_table_bytes(a.I.n, a.I.mp, a.I.npairs, lb2) + 4 * lds_wave_bytes(a.I.n);
+    hipFuncSetAttribute((const void *)k_bounds<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_bounds<M>, dim3((count + 3) / 4), dim3(256), lds, s, a, which, count, perms, l1, l2,
+                       arg, o1, o2);
+    return hipGetLastError();
+}
+
+#ifdef PBB_M
+template hipError_t launch_explore_m<PBB_M>(const KArgs &, int, hipStream_t);
+template hipError_t occupancy_m<PBB_M>(const KArgs &, int *);
+template hipError_t launch_root_m<PBB_M>(const KArgs &, const int *, hipStream_t);
+template hipError_t launch_bounds_m<PBB_M>(int, const KArgs &, int, const int *, const int *, const int *,
+                                          int, int *, int *, hipStream_t);
+#endif
+
+}  // namespace pbb

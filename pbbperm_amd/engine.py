@@ -1,0 +1,300 @@
+"""Host-side mirror of the reference's GPU worker engine and bound library.
+
+* :class:`GpuBB` follows ``gpubb`` (gpu/include/gpubb.h:24-173) as driven by
+  ``worker_gpu`` (distributed/src/worker_gpu.cpp:26-108): full-interval and
+  factoradic-interval initialisation, ``next`` until all explorers are idle,
+  interval export, incumbent in/out, interruption.
+* :class:`Bounds` follows ``bound_abstract`` (bounds/include/bound_abstract.h:6-34),
+  batched over device buffers.
+* :func:`load_config` reads a gpuconfig.ini with the reference's keys
+  (common/src/arguments.cpp:68-133).
+
+Every call goes through the C ABI of ``libpbb_engine.so``; there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, PbbError, Stats, check, load
+
+INT_MAX = 2**31 - 1
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(_lib.IP)
+
+
+def _i32(a: np.ndarray):
+    return a.ctypes.data_as(_lib.I32P)
+
+
+# --------------------------------------------------------------------- config
+
+def default_config(**overrides) -> Config:
+    cfg = Config()
+    check(load().pbb_config_default(C.byref(cfg)))
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, int(v))
+    return cfg
+
+
+def load_config(path: str) -> Tuple[Config, str, str]:
+    """gpuconfig.ini -> (config, problem, instance)."""
+    cfg = Config()
+    prob = C.create_string_buffer(64)
+    inst = C.create_string_buffer(64)
+    check(load().pbb_config_load_ini(path.encode(), C.byref(cfg), prob, inst))
+    return cfg, prob.value.decode(), inst.value.decode()
+
+
+# ------------------------------------------------------------------ instances
+
+def taillard(ta_id: int) -> np.ndarray:
+    """Processing times (m x n, machine-major) of Taillard instance ta<id>."""
+    buf = np.zeros(20 * 500, dtype=np.int32)
+    n = C.c_int()
+    m = C.c_int()
+    check(load().pbb_instance_taillard(ta_id, C.byref(n), C.byref(m), _i32(buf)))
+    return buf[: n.value * m.value].reshape(m.value, n.value).copy()
+
+
+def synth(seed: int, n: int, m: int) -> np.ndarray:
+    """Seeded synthetic instance, times uniform in [1, 99]."""
+    buf = np.zeros(n * m, dtype=np.int32)
+    check(load().pbb_instance_synth(seed, n, m, _i32(buf)))
+    return buf.reshape(m, n)
+
+
+def synth_shape(seed: int, nlo: int, nhi: int, mchoices: Sequence[int]) -> Tuple[int, int]:
+    mc = np.asarray(mchoices, dtype=np.intc)
+    n = C.c_int()
+    m = C.c_int()
+    check(load().pbb_instance_synth_shape(seed, nlo, nhi, _ip(mc), len(mc), C.byref(n), C.byref(m)))
+    return n.value, m.value
+
+
+def read_vrf(path: str) -> np.ndarray:
+    buf = np.zeros(64 * 1000, dtype=np.int32)
+    n = C.c_int()
+    m = C.c_int()
+    check(load().pbb_instance_read_vrf(path.encode(), C.byref(n), C.byref(m), _i32(buf), buf.size))
+    return buf[: n.value * m.value].reshape(m.value, n.value).copy()
+
+
+def neh(ptm: np.ndarray, start: Optional[Sequence[int]] = None) -> Tuple[int, np.ndarray]:
+    ptm = np.ascontiguousarray(ptm, dtype=np.int32)
+    m, n = ptm.shape
+    perm = np.arange(n, dtype=np.intc) if start is None else np.ascontiguousarray(start, np.intc)
+    cost = C.c_int()
+    check(load().pbb_neh(_i32(ptm), n, m, _ip(perm), C.byref(cost)))
+    return cost.value, perm
+
+
+def makespan(ptm: np.ndarray, perm: Sequence[int]) -> int:
+    ptm = np.ascontiguousarray(ptm, dtype=np.int32)
+    m, n = ptm.shape
+    p = np.ascontiguousarray(perm, np.intc)
+    cost = C.c_int()
+    check(load().pbb_makespan_host(_i32(ptm), n, m, _ip(p), C.byref(cost)))
+    return cost.value
+
+
+# ------------------------------------------------------------------ bounds
+
+class Bounds:
+    """bound_fsp_weak / bound_fsp_strong for one instance, batched on the GPU.
+
+    Inputs are int32 CUDA tensors (torch): perms [B, n], l1 [B], l2 [B].
+    """
+
+    def __init__(self, ptm: np.ndarray):
+        ptm = np.ascontiguousarray(ptm, dtype=np.int32)
+        self.m, self.n = ptm.shape
+        self._h = C.c_void_p()
+        check(load().pbb_bounds_create(_i32(ptm), self.n, self.m, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().pbb_bounds_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _stream():
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _check_inputs(self, perms, l1=None, l2=None):
+        import torch
+        for t in (perms, l1, l2):
+            if t is None:
+                continue
+            if not (t.is_cuda and t.dtype == torch.int32 and t.is_contiguous()):
+                raise ValueError("bound inputs must be contiguous int32 CUDA tensors")
+        if perms.dim() != 2 or perms.shape[1] != self.n:
+            raise ValueError(f"perms must be [B, {self.n}]")
+        B = perms.shape[0]
+        if l1 is not None and (l1.numel() != B or l2.numel() != B):
+            raise ValueError("l1/l2 must have one entry per schedule")
+        return B
+
+    def lb1_children(self, perms, l1, l2, both: bool = True):
+        import torch
+        B = self._check_inputs(perms, l1, l2)
+        cb = torch.empty((B, self.n), dtype=torch.int32, device=perms.device)
+        ce = torch.empty((B, self.n), dtype=torch.int32, device=perms.device) if both else None
+        check(load().pbb_lb1_children_batch(self._h, B, perms.data_ptr(), l1.data_ptr(), l2.data_ptr(),
+                                            int(both), cb.data_ptr(), ce.data_ptr() if both else None,
+                                            self._stream()))
+        return cb, ce
+
+    def lb1_node(self, perms, l1, l2):
+        import torch
+        B = self._check_inputs(perms, l1, l2)
+        out = torch.empty(B, dtype=torch.int32, device=perms.device)
+        check(load().pbb_lb1_node_batch(self._h, B, perms.data_ptr(), l1.data_ptr(), l2.data_ptr(),
+                                        out.data_ptr(), self._stream()))
+        return out
+
+    def lb2_node(self, perms, l1, l2, pairs: int = 0):
+        import torch
+        B = self._check_inputs(perms, l1, l2)
+        out = torch.empty(B, dtype=torch.int32, device=perms.device)
+        check(load().pbb_lb2_node_batch(self._h, B, perms.data_ptr(), l1.data_ptr(), l2.data_ptr(),
+                                        pairs, out.data_ptr(), self._stream()))
+        return out
+
+    def makespan(self, perms):
+        import torch
+        B = self._check_inputs(perms)
+        out = torch.empty(B, dtype=torch.int32, device=perms.device)
+        check(load().pbb_makespan_batch(self._h, B, perms.data_ptr(), out.data_ptr(), self._stream()))
+        return out
+
+
+# ------------------------------------------------------------------ engine
+
+@dataclass
+class SolveStats:
+    decomposed: int
+    leaves: int
+    simple_bounds: int
+    johnson_bounds: int
+    explorers: int
+    active: int
+    kernel_ms: float
+    depth_histo: List[int] = field(default_factory=list)
+
+
+class GpuBB:
+    """The IVM branch-and-bound engine on one GPU (gpubb)."""
+
+    def __init__(self, ptm: np.ndarray, config: Optional[Config] = None, ub: int = INT_MAX,
+                 ub_perm: Optional[Sequence[int]] = None, root: Optional[Sequence[int]] = None):
+        ptm = np.ascontiguousarray(ptm, dtype=np.int32)
+        self.m, self.n = ptm.shape
+        self.ptm = ptm
+        self.config = config if config is not None else default_config()
+        up = None if ub_perm is None else np.ascontiguousarray(ub_perm, np.intc)
+        self._h = C.c_void_p()
+        check(load().pbb_create(_i32(ptm), self.n, self.m, C.byref(self.config), int(ub),
+                                None if up is None else _ip(up), C.byref(self._h)))
+        if root is not None:
+            r = np.ascontiguousarray(root, np.intc)
+            check(load().pbb_set_root(self._h, _ip(r)))
+
+    def close(self):
+        if self._h:
+            load().pbb_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # gpubb::initFullInterval
+    def init_full_interval(self):
+        check(load().pbb_init_full(self._h))
+
+    # gpubb::initFromFac
+    def init_from_fac(self, ids: Sequence[int], pos: np.ndarray, end: np.ndarray):
+        ids = np.ascontiguousarray(ids, np.intc)
+        pos = np.ascontiguousarray(pos, np.intc).reshape(-1)
+        end = np.ascontiguousarray(end, np.intc).reshape(-1)
+        nb = len(ids)
+        if pos.size != nb * self.n or end.size != nb * self.n:
+            raise ValueError("pos/end must hold nb*n factoradic digits")
+        check(load().pbb_init_intervals(self._h, nb, _ip(ids), _ip(pos), _ip(end)))
+
+    # gpubb::next
+    def next(self, node_budget: int = 0, time_budget: float = 0.0) -> bool:
+        done = C.c_int(0)
+        check(load().pbb_run(self._h, int(node_budget), float(time_budget), C.byref(done)))
+        return bool(done.value)
+
+    # gpubb::getIntervals
+    def get_intervals(self):
+        E = self.stats().explorers
+        pos = np.zeros(E * self.n, np.intc)
+        end = np.zeros(E * self.n, np.intc)
+        ids = np.zeros(E, np.intc)
+        nb = C.c_int()
+        dec = C.c_uint64()
+        lv = C.c_uint64()
+        check(load().pbb_get_intervals(self._h, _ip(pos), _ip(end), _ip(ids), C.byref(nb), E,
+                                       C.byref(dec), C.byref(lv)))
+        k = nb.value
+        return (ids[:k].copy(), pos[: k * self.n].reshape(k, self.n).copy(),
+                end[: k * self.n].reshape(k, self.n).copy(), dec.value, lv.value)
+
+    def interrupt_exploration(self):
+        check(load().pbb_interrupt(self._h))
+
+    def best(self) -> Tuple[int, np.ndarray]:
+        cost = C.c_int()
+        perm = np.zeros(self.n, np.intc)
+        check(load().pbb_get_incumbent(self._h, C.byref(cost), _ip(perm)))
+        return cost.value, perm
+
+    def set_best(self, cost: int, perm: Optional[Sequence[int]] = None):
+        p = None if perm is None else np.ascontiguousarray(perm, np.intc)
+        check(load().pbb_set_incumbent(self._h, int(cost), None if p is None else _ip(p)))
+
+    def stats(self) -> SolveStats:
+        st = Stats()
+        check(load().pbb_get_stats(self._h, C.byref(st)))
+        return SolveStats(st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds, st.explorers,
+                          st.active, st.kernel_ms, list(st.depth_histo)[: self.n + 1])
+
+    def solve(self, node_budget: int = 0, time_budget: float = 0.0) -> bool:
+        """init_full_interval + next until done (or budget). Returns True when finished."""
+        self.init_full_interval()
+        return self.next(node_budget, time_budget)
+
+
+def explorers_for(n: int, m: int, config: Optional[Config] = None) -> int:
+    cfg = config if config is not None else default_config()
+    out = C.c_int()
+    check(load().pbb_query_explorers(n, m, C.byref(cfg), C.byref(out)))
+    return out.value

@@ -1,0 +1,95 @@
+"""Writes the golden fixtures of tests/golden from the reference's own code.
+
+Runs oracle/_ref/ref_harness (the reference's bounds/, multicore/ivm*,
+common/ and heuristic/ sources compiled unmodified by oracle/Makefile, with a
+restated sequentialbb driver loop) in this container, where the reference
+tree exists.  The fixtures are plain data (inputs and expected outputs):
+
+  bounds_<seed>_<n>x<m>.npz   LB1 children (both directions), LB1', LB2
+                              (all pairs, with the idle sum), makespan on
+                              seeded random partial schedules
+  solve_cases.json            optimum / decomposed / leaves / LB1 bounds /
+                              refinement bounds of single-thread solves
+  taillard_ptm.json           sha256 of the ta001..ta120 processing times
+
+Usage: python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle as O  # noqa: E402
+
+BOUND_SHAPES = [(0, 20, 5, 3000), (1, 20, 10, 3000), (2, 50, 20, 600), (3, 9, 4, 3000), (6, 30, 17, 600)]
+
+# (seed, n, m, bound_mode, branching, refine, johnson_pairs, ub_kind)
+SOLVE_CASES = []
+for s in range(24):
+    n, m = 8 + s % 5, 3 + s % 4
+    for (bm, br, rf, jp) in [(0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, -1, 0, 0), (2, 1, 0, 0),
+                             (2, 3, 1, 0), (2, 2, 1, 1), (1, 1, 0, 0), (1, 3, 1, 2)]:
+        SOLVE_CASES.append((9000 + s, n, m, bm, br, rf, jp, "opt"))
+        SOLVE_CASES.append((9000 + s, n, m, bm, br, rf, jp, "neh"))
+for (bm, br, rf, jp) in [(0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0)]:
+    SOLVE_CASES.append((0, 20, 5, bm, br, rf, jp, "neh"))
+    SOLVE_CASES.append((0, 20, 5, bm, br, rf, jp, "opt"))
+for (bm, br, rf, jp) in [(2, 1, 1, 0), (2, 2, 1, 0), (2, 3, 1, 0), (2, 1, 0, 0), (0, 3, 0, 0)]:
+    SOLVE_CASES.append((1, 20, 10, bm, br, rf, jp, "opt"))
+SOLVE_CASES.append((42, 16, 8, 0, 1, 0, 0, "opt"))
+SOLVE_CASES.append((3, 16, 6, 2, 3, 1, 0, "opt"))
+
+
+def ref_bounds_fixture(seed, n, m, count):
+    ptm = O.synth_ptm(seed, n, m)
+    rows = O.ref_bounds(ptm, count, 77 + seed, pairs=0)
+    arr = lambda i: np.array([r[i] for r in rows], dtype=np.int32)
+    path = os.path.join(HERE, f"bounds_{seed}_{n}x{m}.npz")
+    np.savez_compressed(path, ptm=ptm, l1=arr(0), l2=arr(1), perm=arr(2), cb=arr(3), ce=arr(4),
+                        lb1n=arr(5), lb2=arr(6), idle=arr(7), mk=arr(8))
+    return path
+
+
+def main():
+    if not O.have_ref():
+        O.build(ref=True)
+    for seed, n, m, count in BOUND_SHAPES:
+        print("bounds", seed, n, m, ref_bounds_fixture(seed, n, m, count), flush=True)
+
+    cases = []
+    opt_cache = {}
+    for (seed, n, m, bm, br, rf, jp, ubk) in SOLVE_CASES:
+        ptm = O.synth_ptm(seed, n, m)
+        neh_ref_free, _ = O.Instance(ptm).neh()
+        if (seed, n, m) not in opt_cache:
+            opt_cache[(seed, n, m)] = O.ref_solve(ptm, neh_ref_free, 0, 1).best
+        ub = opt_cache[(seed, n, m)] if ubk == "opt" else neh_ref_free
+        r = O.ref_solve(ptm, ub, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+        cases.append(dict(seed=seed, n=n, m=m, bound_mode=bm, branching=br, refine=rf,
+                          johnson_pairs=jp, ub=ub, ub_kind=ubk, best=r.best, decomposed=r.decomposed,
+                          leaves=r.leaves, simple_bounds=r.simple_bounds,
+                          johnson_bounds=r.johnson_bounds))
+    with open(os.path.join(HERE, "solve_cases.json"), "w") as f:
+        json.dump(cases, f, indent=0)
+    print("solve cases", len(cases), flush=True)
+
+    tai = {}
+    for i in range(1, 121):
+        ptm = O.ref_taillard(i)
+        tai[str(i)] = dict(n=int(ptm.shape[1]), m=int(ptm.shape[0]),
+                           sha256=hashlib.sha256(ptm.astype("<i4").tobytes()).hexdigest())
+    with open(os.path.join(HERE, "taillard_ptm.json"), "w") as f:
+        json.dump(tai, f, indent=0)
+    print("taillard", len(tai), flush=True)
+
+
+if __name__ == "__main__":
+    main()

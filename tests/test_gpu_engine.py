@@ -1,0 +1,253 @@
+"""GPU engine parity with the CPU oracle (sequentialbb semantics).
+
+With the incumbent frozen at the optimum (every leaf of cost >= UB is pruned,
+so the UB never moves), the set of decomposed nodes is independent of how
+the interval is split among explorers, and the engine must reproduce the
+oracle's optimum, decomposed-node, leaf, LB1-bound and refinement-bound
+counts exactly.  With an improving incumbent only the optimum is fixed.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+INT_MAX = 2**31 - 1
+
+# (bound_mode, branching, refine, johnson_pairs)
+MODES = [
+    (0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, -1, 0, 0),
+    (2, 1, 0, 0), (2, 3, 1, 0), (2, 2, 1, 1), (2, 1, 1, 2),
+    (1, 1, 0, 0), (1, 3, 1, 0), (0, -2, 0, 0), (0, -3, 0, 0),
+]
+
+
+def _cfg(P, bm, br, rf, jp, nb=4096, **kw):
+    return P.default_config(bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp,
+                            early_stop_johnson=0, nb_ivms=nb, **kw)
+
+
+def _gpu(P, ptm, ub, bm, br, rf, jp, nb=4096, **kw):
+    eng = P.GpuBB(ptm, _cfg(P, bm, br, rf, jp, nb, **kw), ub=ub)
+    done = eng.solve(time_budget=60.0)
+    assert done
+    st = eng.stats()
+    best, perm = eng.best()
+    eng.close()
+    return best, perm, st
+
+
+def _opt(O, ptm):
+    inst = O.Instance(ptm)
+    c, _ = inst.neh()
+    return O.solve(ptm, c, bound_mode=0, branching=1).best
+
+
+def _same_counts(st, ref):
+    return (st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds) == \
+        (ref.decomposed, ref.leaves, ref.simple_bounds, ref.johnson_bounds)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_small_instances_all_modes_frozen_ub(oracle_mod, seed):
+    import pbbperm_amd as P
+    O = oracle_mod
+    n, m = 8 + seed % 6, 3 + seed % 4
+    ptm = O.synth_ptm(5000 + seed, n, m)
+    opt = _opt(O, ptm)
+    for (bm, br, rf, jp) in MODES:
+        ref = O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+        best, _, st = _gpu(P, ptm, opt, bm, br, rf, jp)
+        print(f"seed={seed} n={n} m={m} mode={(bm, br, rf, jp)} dec={st.decomposed}/{ref.decomposed}",
+              flush=True)
+        assert best == ref.best == opt
+        assert _same_counts(st, ref), (st, ref)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_small_instances_improving_ub(oracle_mod, seed):
+    """UB from NEH (and from infinity): the optimum and a permutation realising it."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    n, m = 9 + seed % 4, 4 + seed % 3
+    ptm = O.synth_ptm(7000 + seed, n, m)
+    inst = O.Instance(ptm)
+    neh, _ = inst.neh()
+    for ub in (neh, INT_MAX):
+        for (bm, br, rf, jp) in [(0, 1, 0, 0), (2, 3, 1, 0), (0, 2, 0, 0)]:
+            ref = O.solve(ptm, ub, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+            best, perm, _ = _gpu(P, ptm, ub, bm, br, rf, jp)
+            assert best == ref.best
+            if best < ub:
+                assert sorted(perm.tolist()) == list(range(n))
+                assert inst.makespan(perm) == best
+
+
+def test_explorer_count_invariance(oracle_mod):
+    """Work stealing between explorers must not change the counts (frozen UB)."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(1, 20, 10)
+    opt = _opt(O, ptm)
+    ref = O.solve(ptm, opt, bound_mode=0, branching=3)
+    for nb in (1, 2, 64, 512, 100000):
+        best, _, st = _gpu(P, ptm, opt, 0, 3, 0, 0, nb=nb)
+        print(f"nb_ivms={nb} explorers={st.explorers} dec={st.decomposed}", flush=True)
+        assert best == opt and _same_counts(st, ref)
+
+
+def test_config1_20x5_lb1_neh(oracle_mod):
+    """20 jobs x 5 machines, seed 0, LB1, UB from NEH, 4096 explorers."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(0, 20, 5)
+    neh, _ = O.Instance(ptm).neh()
+    assert P.neh(ptm)[0] == neh
+    ref = O.solve(ptm, neh, bound_mode=0, branching=1)
+    best, perm, st = _gpu(P, ptm, neh, 0, 1, 0, 0, nb=4096)
+    assert best == ref.best
+    assert P.makespan(ptm, perm) == best
+    ref_opt = O.solve(ptm, ref.best, bound_mode=0, branching=1)
+    _, _, st2 = _gpu(P, ptm, ref.best, 0, 1, 0, 0, nb=4096)
+    assert _same_counts(st2, ref_opt)
+
+
+@pytest.mark.parametrize("branching", [1, 2, 3])
+def test_config2_20x10_lb1_johnson(oracle_mod, branching):
+    """20 jobs x 10 machines, seed 1, LB1 + Johnson LB2 over all pairs, UB = optimum."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(1, 20, 10)
+    opt = _opt(O, ptm)
+    ref = O.solve(ptm, opt, bound_mode=2, branching=branching, refine=1, johnson_pairs=0)
+    best, _, st = _gpu(P, ptm, opt, 2, branching, 1, 0, nb=16384)
+    print(f"branching={branching} dec={st.decomposed} ref={ref.decomposed} jb={st.johnson_bounds}")
+    assert best == opt and _same_counts(st, ref)
+
+
+def test_depth_histogram(oracle_mod):
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(11, 14, 5)
+    opt = _opt(O, ptm)
+    inst = O.Instance(ptm)
+    bb = O.BB(inst, opt, bound_mode=0, branching=1)
+    bb.init_full()
+    bb.run()
+    ref = bb.depth_histo().tolist()
+    _, _, st = _gpu(P, ptm, opt, 0, 1, 0, 0)
+    assert st.depth_histo == ref
+
+
+def test_budgeted_runs_resume(oracle_mod):
+    """next() with small node budgets, repeated until done, equals one run."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(3, 16, 6)
+    opt = _opt(O, ptm)
+    ref = O.solve(ptm, opt, bound_mode=2, branching=3, refine=1)
+    eng = P.GpuBB(ptm, _cfg(P, 2, 3, 1, 0), ub=opt)
+    eng.init_full_interval()
+    calls = 0
+    while not eng.next(node_budget=2000, time_budget=60.0):
+        calls += 1
+        assert calls < 10000
+    st = eng.stats()
+    eng.close()
+    assert calls > 0
+    assert _same_counts(st, ref)
+
+
+def test_factoradic_intervals_import(oracle_mod):
+    """initFromFac with the full interval cut in pieces: each piece is unfolded
+    (bounded, uncounted); counts equal the oracle summed over the pieces."""
+    import pbbperm_amd as P
+    from pbbperm_amd import factoradic as F
+    O = oracle_mod
+    n, m = 16, 8
+    ptm = O.synth_ptm(42, n, m)
+    opt = _opt(O, ptm)
+    pos, end = F.split(n, 37)
+    inst = O.Instance(ptm)
+    tot = [0, 0, 0]
+    for p, e in zip(pos, end):
+        bb = O.BB(inst, opt, bound_mode=0, branching=1)
+        bb.init_interval(p, e)
+        bb.run()
+        st = bb.stats()
+        tot[0] += st.decomposed
+        tot[1] += st.leaves
+        tot[2] += st.simple_bounds
+    eng = P.GpuBB(ptm, _cfg(P, 0, 1, 0, 0, nb=256), ub=opt)
+    eng.init_from_fac(np.arange(len(pos)), pos, end)
+    assert eng.next(time_budget=60.0)
+    st = eng.stats()
+    eng.close()
+    assert [st.decomposed, st.leaves, st.simple_bounds] == tot
+
+
+def test_export_import_intervals(oracle_mod):
+    """Interrupted run -> getIntervals -> fresh engine initFromFac: the node
+    total equals the first part plus the oracle over the exported pieces."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    n, m = 16, 6
+    ptm = O.synth_ptm(3, n, m)
+    opt = _opt(O, ptm)
+    inst = O.Instance(ptm)
+    eng = P.GpuBB(ptm, _cfg(P, 0, 1, 0, 0, nb=128), ub=opt)
+    eng.init_full_interval()
+    done = eng.next(node_budget=20000, time_budget=60.0)
+    assert not done
+    ids, pos, end, dec1, _ = eng.get_intervals()
+    eng.close()
+    assert len(ids) > 0
+    expect = dec1
+    for p, e in zip(pos, end):
+        bb = O.BB(inst, opt, bound_mode=0, branching=1)
+        bb.init_interval(p, e)
+        bb.run()
+        expect += bb.stats().decomposed
+    eng2 = P.GpuBB(ptm, _cfg(P, 0, 1, 0, 0, nb=256), ub=opt)
+    eng2.init_from_fac(np.arange(len(ids)), pos, end)
+    assert eng2.next(time_budget=60.0)
+    assert dec1 + eng2.stats().decomposed == expect
+    eng2.close()
+
+
+def test_interrupt_and_continue(oracle_mod):
+    import time
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(2, 50, 20)
+    neh, _ = O.Instance(ptm).neh()
+    eng = P.GpuBB(ptm, _cfg(P, 0, 1, 0, 0), ub=neh)
+    eng.init_full_interval()
+    t = threading.Timer(0.5, eng.interrupt_exploration)
+    t.start()
+    t0 = time.time()
+    done = eng.next(time_budget=30.0)
+    el = time.time() - t0
+    t.join()
+    assert not done and el < 10.0
+    st = eng.stats()
+    assert st.decomposed > 0 and st.active > 0
+    assert not eng.next(time_budget=0.3)
+    assert eng.stats().decomposed > st.decomposed
+    eng.close()
+
+
+def test_engine_rejects_bad_config():
+    import pbbperm_amd as P
+    ptm = P.synth(0, 10, 3)
+    with pytest.raises(P.PbbError):
+        P.GpuBB(ptm, P.default_config(sorted_dfs=1))
+    with pytest.raises(P.PbbError):
+        P.GpuBB(ptm, P.default_config(johnson_pairs=3))
+    eng = P.GpuBB(ptm, P.default_config(nb_ivms=8))
+    with pytest.raises(P.PbbError):
+        eng.init_from_fac([0], np.full((1, 10), 9), np.zeros((1, 10)))
+    with pytest.raises(P.PbbError):
+        eng.next()
+    eng.close()

@@ -1,0 +1,132 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports
+every entry point include/pbb_engine.h declares, the gpuconfig.ini surface,
+the instance generators / readers, NEH and the factoradic codec agree with
+the oracle and the reference's conventions."""
+import ctypes as C
+import os
+import re
+from math import factorial
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_symbols():
+    with open(os.path.join(ROOT, "include", "pbb_engine.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\*?\s*(pbb_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    import pbbperm_amd as P
+    lib = P.load()
+    syms = _header_symbols()
+    assert len(syms) >= 28
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert lib.pbb_version().decode().startswith("pbbperm_amd")
+
+
+def test_python_bindings_cover_header():
+    from pbbperm_amd import _lib
+    assert set(_header_symbols()) == set(_lib.SIGNATURES)
+
+
+def test_config_defaults_match_reference_defaults():
+    import pbbperm_amd as P
+    c = P.default_config()
+    # common/src/arguments.cpp:135-175
+    assert (c.bound_mode, c.branching, c.johnson_pairs, c.early_stop_johnson, c.nb_ivms) == (2, 3, 1, 1, 4096)
+    assert c.sorted_dfs == 0 and c.find_all == 0
+
+
+def test_ini_surface(tmp_path):
+    import pbbperm_amd as P
+    ini = tmp_path / "gpuconfig.ini"
+    ini.write_text(
+        "[problem]\nproblem = flowshop ;\ninstance = ta20 ;\n"
+        "[initial]\nub = 0\n"
+        "[bb]\nsingleNode = true\nsortedDFS = 0\nboundingMode = 2\n"
+        "adaptiveBranchingMode = 1 ; //3\nJohnsonMode = 0  ; [0] full\nearlyStopJohnson = false\n"
+        "findAll = false ;\n"
+        "[gpu]\nnbIVMs = 16384\n[time]\nbalance = 5\n")
+    cfg, prob, inst = P.load_config(str(ini))
+    assert (prob, inst) == ("flowshop", "ta20")
+    d = cfg.as_dict()
+    assert d["bound_mode"] == 2 and d["branching"] == 1 and d["johnson_pairs"] == 0
+    assert d["early_stop_johnson"] == 0 and d["single_node"] == 1 and d["nb_ivms"] == 16384
+    assert d["init_mode"] == 0 and d["balance_s"] == 5
+    # sortedDFS defaults to 1 when missing, as in the reference; the engine rejects it
+    bad = tmp_path / "bad.ini"
+    bad.write_text("[bb]\nboundingMode = 0\n")
+    with pytest.raises(P.PbbError):
+        P.load_config(str(bad))
+    bad.write_text("[bb]\nsortedDFS = 0\nJohnsonMode = 3\n")
+    with pytest.raises(P.PbbError):
+        P.load_config(str(bad))
+
+
+def test_shipped_example_config_loads():
+    import pbbperm_amd as P
+    cfg, prob, inst = P.load_config(os.path.join(ROOT, "pbbperm_amd", "gpuconfig.ini"))
+    assert prob == "flowshop" and cfg.sorted_dfs == 0
+
+
+def test_generators_match_oracle(oracle_mod):
+    import pbbperm_amd as P
+    for i in (1, 11, 21, 31, 51, 81, 111):
+        assert np.array_equal(P.taillard(i), oracle_mod.taillard(i))
+    for s in range(5):
+        assert np.array_equal(P.synth(s, 20, 7), oracle_mod.synth_ptm(s, 20, 7))
+        assert P.synth_shape(100 + s, 10, 22, [5, 10]) == oracle_mod.synth_shape(100 + s, 10, 22, [5, 10])
+    ptm = P.synth(0, 20, 5)
+    assert ptm.min() >= 1 and ptm.max() <= 99
+
+
+def test_neh_and_makespan_match_oracle(oracle_mod):
+    import pbbperm_amd as P
+    for s, n, m in [(0, 20, 5), (1, 20, 10), (2, 50, 20), (9, 7, 3)]:
+        ptm = P.synth(s, n, m)
+        inst = oracle_mod.Instance(ptm)
+        c, perm = P.neh(ptm)
+        oc, operm = inst.neh()
+        assert c == oc and list(perm) == list(operm)
+        assert P.makespan(ptm, perm) == inst.makespan(perm) == c
+
+
+def test_vrf_reader(tmp_path):
+    import pbbperm_amd as P
+    ptm = P.synth(3, 9, 4)
+    path = tmp_path / "VSYN_3_9x4_Gap.txt"
+    lines = [f"{ptm.shape[1]} {ptm.shape[0]}"]
+    for j in range(ptm.shape[1]):
+        lines.append("  ".join(f"{k} {ptm[k, j]}" for k in range(ptm.shape[0])))
+    path.write_text("\n".join(lines) + "\n")
+    assert np.array_equal(P.read_vrf(str(path)), ptm)
+
+
+def test_factoradic_codec():
+    from pbbperm_amd import factoradic as F
+    for n in (3, 7, 20, 50):
+        for v in (0, 1, factorial(n) - 1, factorial(n) // 3):
+            assert F.to_int(F.to_digits(v, n)) == v
+        p, e = F.full_interval(n)
+        assert F.to_int(p) == 0 and F.to_int(e) == factorial(n) - 1
+    pos, end = F.split(10, 7)
+    assert F.to_int(pos[0]) == 0 and F.to_int(end[-1]) == factorial(10) - 1
+    for k in range(1, 7):
+        assert F.to_int(pos[k]) == F.to_int(end[k - 1]) + 1
+    assert sum(F.length(p, e) for p, e in zip(pos, end)) == factorial(10)
+    with pytest.raises(ValueError):
+        F.to_int([3, 0, 0])
+
+
+def test_engine_fails_loudly_without_device():
+    import torch
+    import pbbperm_amd as P
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(P.PbbError):
+        P.GpuBB(P.synth(0, 10, 3))
