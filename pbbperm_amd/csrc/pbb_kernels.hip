@@ -146,7 +146,7 @@ struct Explorer {
     int F[M], B[M];
     int nf, nb;
     int local_best;
-    unsigned long long dec, dec_flushed, leaves, sb, jb;
+    unsigned dec, leaves, sb, jb;   // deltas since the last flush
     int hist;
     unsigned long long t_end;
 
@@ -156,7 +156,7 @@ struct Explorer {
         L = 0;
         rowv = posv = endv = dirv = remv = 0;
         nf = nb = 0;
-        dec = dec_flushed = leaves = sb = jb = 0;
+        dec = leaves = sb = jb = 0;
         hist = 0;
         local_best = PBB_INF;
         t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
@@ -327,7 +327,7 @@ struct Explorer {
         if constexpr (!REF) {
             int lbF = lb1_front(pj, nullptr);
             int lbB = both ? lb1_back(pj, nullptr) : 0;
-            sb += (unsigned long long)(both ? 2 * k : k);
+            sb += (unsigned)(both ? 2 * k : k);
             dir = choose(valid, lbF, lbB);
             c1 = dir ? lbB : lbF;
         } else {
@@ -343,7 +343,7 @@ struct Explorer {
                 refined = true;
             } else {
                 if (!both) lbB = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
-                sb += (unsigned long long)(both ? 2 * k : k);
+                sb += (unsigned)(both ? 2 * k : k);
                 dir = choose(valid, lbF, lbB);
                 c1 = dir ? lbB : lbF;
                 if (mode == 2) {
@@ -471,7 +471,7 @@ struct Explorer {
                                                 const int *chB, int &csF, int &csB)
     {
         const int k = n - L;
-        jb += 2ull * k;
+        jb += 2u * (unsigned)k;
         if (a.refine == 0) {
             csF = valid ? lb1p_child(0, pj, chF, chB) : 0;
             csB = valid ? lb1p_child(1, pj, chF, chB) : 0;
@@ -676,10 +676,11 @@ struct Explorer {
     // ivm.cpp:188-237): split at the first line l whose pos differs from end
     // (l <= L, l < n - margin); the thief gets the explorable cells after the
     // victim's kept half, rows 0..l and the victim's end; no re-bounding.
-    __device__ __forceinline__ void try_donate()
+    // h, t: idle-ring head/tail loaded before the node's work (prefetch)
+    __device__ __forceinline__ void try_donate(unsigned h, unsigned t)
     {
-        unsigned h = (unsigned)uni((int)ld_rlx(&a.ctl->idle_head));
-        unsigned t = (unsigned)uni((int)ld_rlx(&a.ctl->idle_tail));
+        h = (unsigned)uni((int)h);
+        t = (unsigned)uni((int)t);
         if (h == t) return;
         int lim = min(L, n - a.steal_margin);
         if (lim < 0) lim = 0;
@@ -766,13 +767,12 @@ struct Explorer {
     __device__ __forceinline__ void flush_counters()
     {
         if (lane == 0) {
-            if (dec > dec_flushed) atomicAdd(&a.ctl->dec, dec - dec_flushed);
-            if (leaves) atomicAdd(&a.ctl->leaves, leaves);
-            if (sb) atomicAdd(&a.ctl->sb, sb);
-            if (jb) atomicAdd(&a.ctl->jb, jb);
+            if (dec) atomicAdd(&a.ctl->dec, (unsigned long long)dec);
+            if (leaves) atomicAdd(&a.ctl->leaves, (unsigned long long)leaves);
+            if (sb) atomicAdd(&a.ctl->sb, (unsigned long long)sb);
+            if (jb) atomicAdd(&a.ctl->jb, (unsigned long long)jb);
         }
-        dec_flushed = dec;
-        leaves = sb = jb = 0;
+        dec = leaves = sb = jb = 0;
         if (hist && lane <= n) atomicAdd(&a.ctl->histo[lane], (unsigned long long)hist);
         hist = 0;
     }
@@ -781,11 +781,11 @@ struct Explorer {
     __device__ __forceinline__ bool check_stop()
     {
         if (a.node_budget) {
-            unsigned long long delta = dec - dec_flushed;
-            unsigned long long before = w_add64(&a.ctl->dec, delta);
-            dec_flushed = dec;
-            if (before + delta >= a.node_budget && lane == 0) st_rlx(&a.ctl->stop, 1);
+            unsigned long long before = w_add64(&a.ctl->dec, (unsigned long long)dec);
+            if (before + dec >= a.node_budget && lane == 0) st_rlx(&a.ctl->stop, 1);
+            dec = 0;
         }
+        if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
         if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(&a.ctl->stop, 1);
         if (a.host_flag && lane == 0) {
             int hf = __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -840,6 +840,9 @@ __global__ __launch_bounds__(256) void k_explore(KArgs a)
                 __builtin_amdgcn_s_sleep(8);
             }
         }
+        // idle-ring poll issued before the node's work so its latency is hidden
+        const unsigned rh = ld_rlx(&a.ctl->idle_head);
+        const unsigned rt = ld_rlx(&a.ctl->idle_tail);
         if (!X.step()) {
             X.go_idle();
             st = ST_IDLE;
@@ -850,7 +853,7 @@ __global__ __launch_bounds__(256) void k_explore(KArgs a)
             X.flush_counters();
             return;
         }
-        X.try_donate();
+        X.try_donate(rh, rt);
     }
 }
 

@@ -57,12 +57,29 @@ def test_small_instances_all_modes_frozen_ub(oracle_mod, seed):
     ptm = O.synth_ptm(5000 + seed, n, m)
     opt = _opt(O, ptm)
     for (bm, br, rf, jp) in MODES:
+        if br in (-2, -3) and n > 9:
+            continue  # back rows are never pruned in these modes: near-full enumeration
         ref = O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
         best, _, st = _gpu(P, ptm, opt, bm, br, rf, jp)
         print(f"seed={seed} n={n} m={m} mode={(bm, br, rf, jp)} dec={st.decomposed}/{ref.decomposed}",
               flush=True)
         assert best == ref.best == opt
         assert _same_counts(st, ref), (st, ref)
+
+
+@pytest.mark.parametrize("seed,n,m", [(42, 16, 8), (3, 16, 6)])
+def test_larger_trees_frozen_ub(oracle_mod, seed, n, m):
+    """Trees of 10^4-10^6 nodes: work sharing is exercised for real."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(seed, n, m)
+    opt = _opt(O, ptm)
+    for (bm, br, rf, jp) in [(0, 1, 0, 0), (0, 3, 0, 0), (2, 3, 1, 0), (2, 1, 0, 0), (0, 2, 0, 0)]:
+        ref = O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+        best, _, st = _gpu(P, ptm, opt, bm, br, rf, jp)
+        print(f"seed={seed} mode={(bm, br, rf, jp)} dec={st.decomposed}/{ref.decomposed} "
+              f"explorers={st.explorers}", flush=True)
+        assert best == opt and _same_counts(st, ref), (st, ref)
 
 
 @pytest.mark.parametrize("seed", range(6))
