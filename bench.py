@@ -23,6 +23,11 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 256 CUs x 64 int32 lanes/clk x 2.4 GHz
+PEAK_HBM_GBS = 8000.0
+# HBM bytes per decomposed node of k_explore on this workload, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
+# plain 256 MiB device copy): profiles/k_explore_50x20_lb1_pmc.md
+HBM_BYTES_PER_NODE = 5.32
 
 
 def lb1_ops(depth: int, n: int, m: int) -> int:
@@ -60,6 +65,8 @@ def main():
     ap.add_argument("--explorers", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
@@ -69,6 +76,12 @@ def main():
     import numpy as np
     import torch
     import pbbperm_amd as P
+
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
 
     n, m, seed = 50, 20, 2
     ptm = P.synth(seed, n, m)
@@ -97,6 +110,8 @@ def main():
     value = nodes / wall
     ops = sum((st1.depth_histo[d] - st0.depth_histo[d]) * lb1_ops(d, n, m) for d in range(1, n - 1))
     achieved = ops / (kernel_ms / 1e3) / 1e12
+    nodes_per_launch = nodes / args.steps
+    traffic = HBM_BYTES_PER_NODE * nodes_per_launch
     line = {
         "metric": "decomposed nodes/sec (50x20 PFSP, LB1)",
         "value": value,
@@ -115,7 +130,11 @@ def main():
                    "n": n, "m": m, "seed": seed, "explorers": st1.explorers, "ub": int(ub), "best": int(best),
                    "node_budget_per_step": args.node_budget},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s (int32)",
-                     "frac": achieved / PEAK_INT32_TOPS, "traffic": None,
+                     "frac": achieved / PEAK_INT32_TOPS, "traffic": traffic,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+                     "hbm_GBs": traffic / (kernel_ms / args.steps / 1e3) / 1e9,
+                     "hbm_frac": traffic / (kernel_ms / args.steps / 1e3) / 1e9 / PEAK_HBM_GBS,
+                     "kernel": "k_explore<20,false>",
                      "kernel_ms_per_step": kernel_ms / args.steps,
                      "ops_per_step": ops / args.steps},
     }

@@ -38,25 +38,39 @@ __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, PBB_WAVE); }
 
-__device__ __forceinline__ int wave_sum(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, PBB_WAVE);
-    return v;
-}
+// Wavefront reductions on DPP lanes moves (row_shr 1,2,4,8 inside each
+// 16-lane row, row_bcast:15 and row_bcast:31 across rows): the result is
+// complete in lane 63 and read back with v_readlane, without an LDS round
+// trip per step.
+#define PBB_DPP_REDUCE(NAME, OP, IDENT)                                              \
+    __device__ __forceinline__ int NAME(int x)                                       \
+    {                                                                                \
+        x = OP(x, __builtin_amdgcn_update_dpp(IDENT, x, 0x111, 0xF, 0xF, false));   \
+        x = OP(x, __builtin_amdgcn_update_dpp(IDENT, x, 0x112, 0xF, 0xF, false));   \
+        x = OP(x, __builtin_amdgcn_update_dpp(IDENT, x, 0x114, 0xF, 0xF, false));   \
+        x = OP(x, __builtin_amdgcn_update_dpp(IDENT, x, 0x118, 0xF, 0xF, false));   \
+        x = OP(x, __builtin_amdgcn_update_dpp(IDENT, x, 0x142, 0xA, 0xF, false));   \
+        x = OP(x, __builtin_amdgcn_update_dpp(IDENT, x, 0x143, 0xC, 0xF, false));   \
+        return __builtin_amdgcn_readlane(x, 63);                                     \
+    }
 
-__device__ __forceinline__ int wave_min(int v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, PBB_WAVE));
-    return v;
-}
+__device__ __forceinline__ int op_add(int a, int b) { return a + b; }
+__device__ __forceinline__ int op_min(int a, int b) { return min(a, b); }
+__device__ __forceinline__ int op_max(int a, int b) { return max(a, b); }
+PBB_DPP_REDUCE(wave_sum, op_add, 0)
+PBB_DPP_REDUCE(wave_min, op_min, 0x7fffffff)
+PBB_DPP_REDUCE(wave_max, op_max, (int)0x80000000)
+#undef PBB_DPP_REDUCE
 
-__device__ __forceinline__ int wave_max(int v)
+// lanes i and 32+i both receive x[i] (gfx950 v_permlane32_swap)
+__device__ __forceinline__ int bcast_low_half(int x)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, PBB_WAVE));
-    return v;
+    return __builtin_amdgcn_permlane32_swap(x, x, false, false)[0];
+}
+// lanes i and 32+i both receive x[32+i]
+__device__ __forceinline__ int bcast_high_half(int x)
+{
+    return __builtin_amdgcn_permlane32_swap(x, x, false, false)[1];
 }
 
 __device__ __forceinline__ int flag_neg(int j) { return j < 0 ? j : -j - 1; }  // misc.h negative()

@@ -61,25 +61,34 @@ struct KArgs {
     int steal_margin;
 };
 
-// LDS footprint: tables [ptmT n*mp | min_arr | min_dep | tot][Johnson table
-// npairs*n int4 when LB2 is used], then per wavefront an n x 64-byte IVM
-// matrix and 64 ints of scratch.
+// LDS footprint.  Workgroup tables: job-major processing times ptmT[n][mp],
+// the machine-major copy pm[m][n] (stored in an n*mp slot), min_arr, min_dep, per-machine
+// totals, and the Johnson table npairs*n int4 when LB2 is used and it fits.
+// Per wavefront: the IVM matrix (n rows of 64 bytes), the front/back stack
+// ((n+1) x m ints), the LB1 operand block (2 x m int2) and 64 ints of scratch.
 __device__ __host__ inline size_t lds_table_bytes(int n, int mp, int npairs, bool lb2)
 {
-    size_t off = (size_t)(n + 3) * mp * 4;
+    size_t off = (size_t)(2 * n + 3) * mp * 4;
     off = (off + 15) & ~(size_t)15;
     if (lb2) off += (size_t)npairs * n * 16;
     return off;
 }
-__device__ __host__ inline size_t lds_wave_bytes(int n) { return (size_t)n * 64 + 64 * 4; }
+__device__ __host__ inline size_t lds_stack_bytes(int n, int m) { return ((size_t)(n + 1) * m * 4 + 15) & ~(size_t)15; }
+__device__ __host__ inline size_t lds_wave_bytes(int n, int m)
+{
+    return (size_t)n * 64 + lds_stack_bytes(n, m) + (size_t)m * 16 + 64 * 4;
+}
 // The Johnson table is staged in LDS when it is at most 64 KB (m <= 10 at
 // n <= 50, m = 20 at n <= 21); larger tables are read from global memory.
 __device__ __host__ inline bool jtab_in_lds(int n, int npairs) { return (size_t)npairs * n * 16 <= 64 * 1024; }
 inline size_t explore_lds_bytes(const DevInst &I, bool lb2)
 {
-    return lds_table_bytes(I.n, I.mp, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) + 4 * lds_wave_bytes(I.n);
+    return lds_table_bytes(I.n, I.mp, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) + 4 * lds_wave_bytes(I.n, I.m);
 }
-inline size_t root_lds_bytes(const DevInst &I) { return lds_table_bytes(I.n, I.mp, I.npairs, false) + lds_wave_bytes(I.n); }
+inline size_t root_lds_bytes(const DevInst &I)
+{
+    return lds_table_bytes(I.n, I.mp, I.npairs, false) + lds_wave_bytes(I.n, I.m);
+}
 
 // per-M kernels (pbb_kernels.hip, one translation unit per M)
 template <int M> hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s);

@@ -1,19 +1,20 @@
 // pbb_kernels.hip -- gfx950 kernels of the IVM branch-and-bound engine.
 //
 //   k_explore<M,REF> persistent explorer kernel: one IVM per 64-lane
-//                  wavefront, 4 explorers per 256-thread workgroup, the IVM
-//                  matrix of each explorer staged in LDS (int8, 64-byte rows),
-//                  the interval vectors and the front/back stacks in VGPRs.
-//                  Implements sequentialbb::next / unfold
-//                  (multicore/src/sequentialbb.cpp:110-264) with ivm.cpp
-//                  navigation and ivm_bound.cpp bounding/branching/pruning.
-//                  Work is balanced on the device: an explorer whose interval
-//                  is exhausted registers in an idle ring; busy explorers
-//                  split their interval as bbthread::shareWork
-//                  (multicore/src/bbthread.cpp:39-72) and hand the upper part
-//                  to an idle explorer.  No host round trip until the run
-//                  ends (all idle, node budget, deadline or interrupt).
-//                  REF=false compiles the LB1-only path (boundingMode 0).
+//                  wavefront, 4 explorers per 256-thread workgroup.  The IVM
+//                  matrix (int8 rows) and the front/back completion-time
+//                  stacks of each explorer live in LDS; the interval vectors
+//                  and the current row live in VGPRs.  Implements
+//                  sequentialbb::next / unfold (multicore/src/sequentialbb.cpp:110-264)
+//                  with ivm.cpp navigation and ivm_bound.cpp bounding,
+//                  branching and pruning.  Work is balanced on the device: an
+//                  explorer whose interval is exhausted registers in an idle
+//                  ring; busy explorers split their interval as
+//                  bbthread::shareWork (multicore/src/bbthread.cpp:39-72) and
+//                  hand the upper part to an idle explorer.  No host round
+//                  trip until the run ends (all idle, node budget, deadline
+//                  or interrupt).  REF=false compiles the LB1-only path
+//                  (boundingMode 0), REF=true adds the refinement bounds.
 //   k_bound_root<M> LB1 root bounding, ivm_bound::boundRoot (ivm_bound.cpp:273-330).
 //   k_bounds<M>    batched bound_abstract calls over device buffers, one
 //                  partial schedule per wavefront, on the explorer's own
@@ -88,39 +89,77 @@ __device__ __forceinline__ void w_min64(unsigned long long *p, unsigned long lon
     if (lane_id() == 0) __hip_atomic_fetch_min(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// ------------------------------------------------------ lane scans (DPP)
+// Inclusive prefix scans over lanes 0..31 (row_shr 1,2,4,8 inside each
+// 16-lane row, then row_bcast:15 into row 1).  Lanes only combine with lanes
+// to their left, so values in lanes >= the vector length are never read.
+__device__ __forceinline__ int scan_add32(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+    return x;
+}
+__device__ __forceinline__ int scan_max32(int x)
+{
+    const int lo = (int)0x80000000;
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x111, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x112, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x142, 0xA, 0xF, false));
+    return x;
+}
+
+// Max-plus chain of one job over the machines, lane q = machine q:
+// c_q = max(base_q, c_{q-1}) + p_q  ==  P_q + max_{i<=q}(base_i - P_{i-1})
+// with P the prefix sum of p (the recurrence of bound_fsp_weak.cpp:76-82).
+__device__ __forceinline__ int chain_scan(int base, int p)
+{
+    const int P = scan_add32(p);
+    return P + scan_max32(base - (P - p));
+}
+
 // ------------------------------------------------------------ LDS layout
-// [ptmT n*mp][min_arr mp][min_dep mp][tot mp][jtab npairs*n int4 (LB2 only)]
-// then per wave: [matrix n*64 bytes][scratch 64 ints]
 struct Lds {
-    int *ptmT;
+    int *ptmT;      // [n][mp] job-major (uniform per-job rows)
+    int *pm;        // [m][n] machine-major (per-lane loads of different jobs: distinct banks)
     int *minA, *minD, *tot;
-    int4 *jtab;
-    int8_t *mat;   // this wave's matrix, row r at mat + r*64
-    int *scr;      // this wave's scratch
+    int4 *jtab;     // Johnson table (LDS or global)
+    int8_t *mat;    // this wave's IVM matrix, row r at mat + r*64
+    int *stk;       // this wave's stacks: F entry s at row s, B entry s at row n-s ((n+1) x m)
+    int2 *U;        // this wave's LB1 operand block [2][m]
+    int *scr;       // this wave's scratch (64 ints)
 };
 
 __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load_jtab)
 {
     Lds L;
-    const int n = I.n, mp = I.mp;
+    const int n = I.n, mp = I.mp, m = I.m;
     L.ptmT = (int *)smem;
-    L.minA = L.ptmT + n * mp;
+    L.pm = L.ptmT + n * mp;
+    L.minA = L.pm + n * mp;
     L.minD = L.minA + mp;
     L.tot = L.minD + mp;
-    size_t off = (size_t)(n + 3) * mp * 4;
+    size_t off = (size_t)(2 * n + 3) * mp * 4;
     off = (off + 15) & ~(size_t)15;
     load_jtab = load_jtab && jtab_in_lds(n, I.npairs);
     L.jtab = load_jtab ? (int4 *)(smem + off) : const_cast<int4 *>(I.jtab);
     char *wbase = smem + lds_table_bytes(n, mp, I.npairs, load_jtab) +
-                  (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n);
+                  (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n, m);
     L.mat = (int8_t *)wbase;
-    L.scr = (int *)(wbase + (size_t)n * 64);
+    L.stk = (int *)(wbase + (size_t)n * 64);
+    L.U = (int2 *)(wbase + (size_t)n * 64 + lds_stack_bytes(n, m));
+    L.scr = (int *)(wbase + (size_t)n * 64 + lds_stack_bytes(n, m) + (size_t)m * 16);
     for (int i = threadIdx.x; i < n * mp; i += blockDim.x) L.ptmT[i] = I.ptmT[i];
+    for (int i = threadIdx.x; i < n * m; i += blockDim.x) L.pm[i] = I.ptm[i];
     for (int i = threadIdx.x; i < mp; i += blockDim.x) {
-        L.minA[i] = i < I.m ? I.min_arr[i] : 0;
-        L.minD[i] = i < I.m ? I.min_dep[i] : 0;
+        L.minA[i] = i < m ? I.min_arr[i] : 0;
+        L.minD[i] = i < m ? I.min_dep[i] : 0;
         int s = 0;
-        if (i < I.m)
+        if (i < m)
             for (int j = 0; j < n; j++) s += I.ptmT[j * mp + i];
         L.tot[i] = s;
     }
@@ -131,22 +170,23 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
 }
 
 // ------------------------------------------------------------- explorer
-// Register state of one explorer.  Stacks: F[k] lane s = completion time on
-// machine k of the first s front jobs, F[k] lane 0 = min_arr[k] (the front of
-// an empty prefix, bound_fsp_weak.cpp:67-70); B likewise with min_dep.  The
-// chains started from min_arr/min_dep equal those started from zero, since
-// min_arr[k] never exceeds a single job's head (and min_dep its tail).
+// Stacks: F entry s (row s of stk) = completion times on every machine of
+// the first s front jobs, F entry 0 = min_arr (the front of an empty prefix,
+// bound_fsp_weak.cpp:67-70); B entry s (row n-s) = reverse completion times
+// of the last s back jobs, B entry 0 = min_dep.  nf + nb < n, so the two
+// never overlap.  A chain started from min_arr/min_dep equals one started
+// from zero, since min_arr[q] never exceeds a single job's head (and min_dep
+// its tail).
 template <int M, bool REF>
 struct Explorer {
     const KArgs &a;
     Lds lds;
     int n, mp, lane, e;
     int L;                            // current line
-    int rowv, posv, endv, dirv, remv; // lane vectors (remv: lane k = remain on machine k)
-    int F[M], B[M];
+    int rowv, posv, endv, dirv, remv; // lane vectors (remv: lane q = remaining work on machine q)
     int nf, nb;
     int local_best;
-    unsigned dec, leaves, sb, jb;   // deltas since the last flush
+    unsigned dec, leaves, sb, jb;     // deltas since the last flush
     int hist;
     unsigned long long t_end;
 
@@ -160,8 +200,6 @@ struct Explorer {
         hist = 0;
         local_best = PBB_INF;
         t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
-#pragma unroll
-        for (int k = 0; k < M; k++) F[k] = B[k] = 0;
     }
 
     __device__ __forceinline__ bool gt_best(int v) const
@@ -169,7 +207,14 @@ struct Explorer {
         return a.find_all ? v > local_best : v >= local_best;
     }
 
-    __device__ __forceinline__ int pmach(int job) const  // lane k: p[k][job]
+    __device__ __forceinline__ int *frow(int s) const { return lds.stk + s * M; }
+    __device__ __forceinline__ int *brow(int s) const { return lds.stk + (n - s) * M; }
+    // uniform stack values (LDS broadcast reads)
+    __device__ __forceinline__ int fval(int q) const { return frow(nf)[q]; }
+    __device__ __forceinline__ int bval(int q) const { return brow(nb)[q]; }
+    __device__ __forceinline__ int rval(int q) const { return rdlane(remv, q); }
+
+    __device__ __forceinline__ int pmach(int job) const  // lane q: p[q][job]
     {
         return lane < M ? lds.ptmT[job * mp + lane] : 0;
     }
@@ -177,40 +222,34 @@ struct Explorer {
     __device__ __forceinline__ void reset_stacks()
     {
         nf = nb = 0;
-        int ma = lane < M ? lds.minA[lane] : 0;
-        int md = lane < M ? lds.minD[lane] : 0;
-#pragma unroll
-        for (int k = 0; k < M; k++) {
-            F[k] = wrlane(0, rdlane(ma, k), 0);
-            B[k] = wrlane(0, rdlane(md, k), 0);
+        if (lane < M) {
+            frow(0)[lane] = lds.minA[lane];
+            brow(0)[lane] = lds.minD[lane];
         }
         remv = lane < M ? lds.tot[lane] : 0;
+        __builtin_amdgcn_wave_barrier();
     }
 
-    // schedule job j in direction d (push on F or B); remain -= p_j
+    // schedule job j in direction d: push a new F (d=0) or B (d=1) entry,
+    // computed lane-parallel by chain_scan; remain -= p_j.
     __device__ __forceinline__ void push_job(int j, int d)
     {
-        int pk = pmach(j);
+        const int pk = pmach(j);
         if (d == 0) {
-            int c = 0;
-#pragma unroll
-            for (int k = 0; k < M; k++) {
-                int base = rdlane(F[k], nf);
-                c = (k == 0 ? base : max(base, c)) + rdlane(pk, k);
-                F[k] = wrlane(F[k], c, nf + 1);
-            }
+            int base = lane < M ? frow(nf)[lane] : 0;
+            int c = chain_scan(base, pk);
+            if (lane < M) frow(nf + 1)[lane] = c;
             nf++;
         } else {
-            int c = 0;
-#pragma unroll
-            for (int k = M - 1; k >= 0; k--) {
-                int base = rdlane(B[k], nb);
-                c = (k == M - 1 ? base : max(base, c)) + rdlane(pk, k);
-                B[k] = wrlane(B[k], c, nb + 1);
-            }
+            // back chain runs from machine M-1 down: lane q handles machine M-1-q
+            int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
+            int pr = lane < M ? lds.ptmT[j * mp + (M - 1 - lane)] : 0;
+            int c = chain_scan(base, pr);
+            if (lane < M) brow(nb + 1)[M - 1 - lane] = c;
             nb++;
         }
         remv -= pk;
+        __builtin_amdgcn_wave_barrier();
     }
 
     __device__ __forceinline__ void pop_job(int j, int d)
@@ -236,42 +275,70 @@ struct Explorer {
 
     // ------------------------------------------------------- LB1 children
     // bound_fsp_weak::addFrontAndBound / addBackAndBound (bound_fsp_weak.cpp:133-191)
-    // for the job of this lane; the uniform front/back/remain values are
-    // read from the stacks one machine at a time.
-    __device__ __forceinline__ int lb1_front(const int *pj, int *chF) const
+    // for all children of the node, both directions in one pass: lane i < 32
+    // bounds child base+i in the front direction, lane 32+i the same child in
+    // the back direction.  The back recurrence run over machines M-1..0 has
+    // the same form as the front one run over 0..M-1, so both halves execute
+    // one loop:  t1 = max(x_q, t0); lb = max(lb, t1 + y_q); t0 = t1 + p_q,
+    // with (x_q, y_q) = (front_q, remain_q + back_q) in the front half and
+    // (back_{M-1-q}, front_{M-1-q} + remain_{M-1-q}) in the back half, read
+    // from the per-wave operand block as LDS broadcasts.
+    __device__ __forceinline__ void build_operands() const
     {
-        int f0 = rdlane(F[0], nf);
-        int t0 = f0 + pj[0];
-        int lb = f0 + rdlane(remv, 0) + rdlane(B[0], nb);
-        if (chF) chF[0] = t0;
+        if (lane < M) {
+            const int f = frow(nf)[lane], b = brow(nb)[lane], r = remv;
+            lds.U[lane] = make_int2(f, r + b);
+            lds.U[M + (M - 1 - lane)] = make_int2(b, f + r);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // one fused pass over children base..base+31; returns this lane's bound
+    // (chain receives t0 after every step when non-null: the child's own
+    // front chain in machine order (front half) or back chain in reversed
+    // machine order (back half))
+    __device__ __forceinline__ int lb1_pass(int base, int k, int *chain) const
+    {
+        const int half = lane >> 5;
+        const int c = base + (lane & 31);
+        const int job = base == 0 ? bcast_low_half(rowv) : bcast_high_half(rowv);
+        const int jb = c < k ? job : 0;
+        // machine-major rows: front lanes walk machines 0..M-1, back lanes M-1..0
+        const int *pcol = lds.pm + (half ? (M - 1) * n : 0) + jb;
+        const int pstep = half ? -n : n;
+        const int2 *u = lds.U + half * M;
+        int pj[M];
 #pragma unroll
-        for (int k = 1; k < M; k++) {
-            int fk = rdlane(F[k], nf);
-            int rbk = rdlane(remv, k) + rdlane(B[k], nb);
-            int t1 = max(fk, t0);
-            lb = max(lb, t1 + rbk);
-            t0 = t1 + pj[k];
-            if (chF) chF[k] = t0;
+        for (int q = 0; q < M; q++) pj[q] = pcol[q * pstep];
+        int2 o = u[0];
+        int t0 = o.x + pj[0];
+        int lb = o.x + o.y;
+        if (chain) chain[0] = t0;
+#pragma unroll
+        for (int q = 1; q < M; q++) {
+            o = u[q];
+            int t1 = max(o.x, t0);
+            lb = max(lb, t1 + o.y);
+            t0 = t1 + pj[q];
+            if (chain) chain[q] = t0;
         }
         return lb;
     }
 
-    __device__ __forceinline__ int lb1_back(const int *pj, int *chB) const
+    // row layout (lane = child): cb (front) and ce (back) for k <= 64 children
+    __device__ __forceinline__ void lb1_children(int k, int &cb, int &ce, int *chain0) const
     {
-        int bl = rdlane(B[M - 1], nb);
-        int t0 = bl + pj[M - 1];
-        int lb = rdlane(F[M - 1], nf) + rdlane(remv, M - 1) + bl;
-        if (chB) chB[M - 1] = t0;
-#pragma unroll
-        for (int k = M - 2; k >= 0; k--) {
-            int bk = rdlane(B[k], nb);
-            int frk = rdlane(F[k], nf) + rdlane(remv, k);
-            int t1 = max(t0, bk);
-            lb = max(lb, frk + t1);
-            t0 = t1 + pj[k];
-            if (chB) chB[k] = t0;
+        build_operands();
+        int lb0 = lb1_pass(0, k, chain0);
+        if (k <= 32) {
+            cb = lb0;
+            ce = bcast_high_half(lb0);
+        } else {
+            int lb1 = lb1_pass(32, k, nullptr);
+            int sh0 = bcast_high_half(lb0), sh1 = bcast_low_half(lb1);
+            cb = lane < 32 ? lb0 : sh1;
+            ce = lane < 32 ? sh0 : lb1;
         }
-        return lb;
     }
 
     // chooseChildrenSet (ivm_bound.cpp:413-507): lanes beyond the row hold
@@ -316,38 +383,32 @@ struct Explorer {
         const bool valid = lane < k;
         const int job = valid ? rowv : 0;
         const bool both = a.branching > 0;
-        const int *pjp = lds.ptmT + job * mp;
-        int pj[M];
-#pragma unroll
-        for (int q = 0; q < M; q++) pj[q] = pjp[q];
-
-        int dir;
-        int c1, c2 = 0;
+        int dir, c1, c2 = 0;
         bool refined = false;
         if constexpr (!REF) {
-            int lbF = lb1_front(pj, nullptr);
-            int lbB = both ? lb1_back(pj, nullptr) : 0;
+            int cb, ce;
+            lb1_children(k, cb, ce, nullptr);
+            if (!both) ce = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
             sb += (unsigned)(both ? 2 * k : k);
-            dir = choose(valid, lbF, lbB);
-            c1 = dir ? lbB : lbF;
+            dir = choose(valid, cb, ce);
+            c1 = dir ? ce : cb;
         } else {
             const int mode = root ? 0 : a.bound_mode;
-            int chF[M], chB[M];
-            int lbF = lb1_front(pj, chF);
-            int lbB = (both || mode == 1) ? lb1_back(pj, chB) : 0;
+            int cb, ce;
+            lb1_children(k, cb, ce, nullptr);
             if (mode == 1) {
                 int csF = 0, csB = 0;
-                refine_both(valid, job, pj, chF, chB, csF, csB);
+                refine_both(valid, job, csF, csB);
                 dir = choose(valid, csF, csB);
                 c1 = c2 = dir ? csB : csF;
                 refined = true;
             } else {
-                if (!both) lbB = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
+                if (!both) ce = 0;
                 sb += (unsigned)(both ? 2 * k : k);
-                dir = choose(valid, lbF, lbB);
-                c1 = dir ? lbB : lbF;
+                dir = choose(valid, cb, ce);
+                c1 = dir ? ce : cb;
                 if (mode == 2) {
-                    c2 = refine_dir(dir, valid, job, pj, chF, chB, c1);
+                    c2 = refine_dir(dir, valid, job, c1);
                     refined = true;
                 }
             }
@@ -373,16 +434,40 @@ struct Explorer {
         return ((uint64_t)(unsigned)uni((int)hi) << 32) | (unsigned)uni((int)lo);
     }
 
-    // LB1' of this lane's child (bound_fsp_weak::bornes_calculer with the
-    // child's front or back): d=0 front' = chF, d=1 back' = chB.
-    __device__ __forceinline__ int lb1p_child(int d, const int *pj, const int *chF, const int *chB) const
+    // The refinement bounds below work on one child per lane (lane = row
+    // cell), both directions, with the child's chains computed per lane
+    // (the refinement modes serve instances of up to 30 jobs; LB1 above is
+    // the fused hot loop).
+    // d = 0: front' = front + job; d = 1: back' = job + back.
+    __device__ __forceinline__ void child_chains(int job, int (&chF)[M], int (&chB)[M]) const
     {
+        const int *pj = lds.ptmT + job * mp;
+        int t = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int f = fval(q);
+            t = (q == 0 ? f : max(f, t)) + pj[q];
+            chF[q] = t;
+        }
+#pragma unroll
+        for (int q = M - 1; q >= 0; q--) {
+            int b = bval(q);
+            t = (q == M - 1 ? b : max(b, t)) + pj[q];
+            chB[q] = t;
+        }
+    }
+
+    // LB1' of this lane's child (bound_fsp_weak::bornes_calculer with the
+    // child's front or back)
+    __device__ __forceinline__ int lb1p_child(int d, int job, const int (&chF)[M], const int (&chB)[M]) const
+    {
+        const int *pj = lds.ptmT + job * mp;
         int t0 = 0, lb = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
-            int f = d == 0 ? chF[q] : rdlane(F[q], nf);
-            int b = d == 1 ? chB[q] : rdlane(B[q], nb);
-            int r = rdlane(remv, q) - pj[q];
+            int f = d == 0 ? chF[q] : fval(q);
+            int b = d == 1 ? chB[q] : bval(q);
+            int r = rval(q) - pj[q];
             if (q == 0) {
                 t0 = f + r;
                 lb = t0 + b;
@@ -397,22 +482,22 @@ struct Explorer {
 
     // Johnson LB2 of child c (uniform), direction d
     // (bound_fsp_strong::bornes_calculer with the child's partial schedule)
-    __device__ __forceinline__ int lb2_child(int c, int d, const int *chF, const int *chB,
+    __device__ __forceinline__ int lb2_child(int c, int d, const int (&chF)[M], const int (&chB)[M],
                                              uint64_t rowmask, int k)
     {
         const int cj = flag_abs(rdlane(rowv, c));
         if (k <= 2) {
             // l2 - l1 <= 2: makespan of the completed permutation (bound_fsp_strong.cpp:543-546)
-            const int other = rdlane(rowv, 1 - c);
+            const int other = flag_abs(rdlane(rowv, 1 - c));
             int fr[M];
 #pragma unroll
-            for (int q = 0; q < M; q++) fr[q] = d == 0 ? rdlane(chF[q], c) : rdlane(F[q], nf);
-            int po = pmach(flag_abs(other));
+            for (int q = 0; q < M; q++) fr[q] = d == 0 ? rdlane(chF[q], c) : fval(q);
+            int po = pmach(other);
             int cc = 0, best = 0;
 #pragma unroll
             for (int q = 0; q < M; q++) {
                 cc = (q == 0 ? fr[0] : max(fr[q], cc)) + rdlane(po, q);
-                int bk = d == 1 ? rdlane(chB[q], c) : rdlane(B[q], nb);
+                int bk = d == 1 ? rdlane(chB[q], c) : bval(q);
                 best = max(best, cc + bk);
             }
             return best;
@@ -421,8 +506,8 @@ struct Explorer {
         int fl = 0, bl = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
-            int fq = d == 0 ? rdlane(chF[q], c) : rdlane(F[q], nf);
-            int bq = d == 1 ? rdlane(chB[q], c) : rdlane(B[q], nb);
+            int fq = d == 0 ? rdlane(chF[q], c) : fval(q);
+            int bq = d == 1 ? rdlane(chB[q], c) : bval(q);
             if (lane == q) {
                 fl = fq;
                 bl = bq;
@@ -443,15 +528,16 @@ struct Explorer {
 
     // mixed mode (computeStrongBounds, ivm_bound.cpp:141-195): refine the
     // children not pruned by LB1 in the chosen direction; 0 for the others
-    __device__ __forceinline__ int refine_dir(int d, bool valid, int job, const int *pj, const int *chF,
-                                              const int *chB, int cw)
+    __device__ __forceinline__ int refine_dir(int d, bool valid, int job, int cw)
     {
         const int k = n - L;
         bool todo = valid && !gt_best(cw);
         uint64_t tmask = ballot(todo);
-        jb += __popcll(tmask);
+        jb += (unsigned)__popcll(tmask);
+        int chF[M], chB[M];
+        child_chains(job, chF, chB);
         if (a.refine == 0) {
-            int v = lb1p_child(d, pj, chF, chB);
+            int v = lb1p_child(d, job, chF, chB);
             return todo ? v : 0;
         }
         uint64_t rowmask = row_jobs(valid, job);
@@ -467,14 +553,15 @@ struct Explorer {
     }
 
     // strong-only mode (strongBoundPrune, ivm_bound.cpp:668-676)
-    __device__ __forceinline__ void refine_both(bool valid, int job, const int *pj, const int *chF,
-                                                const int *chB, int &csF, int &csB)
+    __device__ __forceinline__ void refine_both(bool valid, int job, int &csF, int &csB)
     {
         const int k = n - L;
         jb += 2u * (unsigned)k;
+        int chF[M], chB[M];
+        child_chains(job, chF, chB);
         if (a.refine == 0) {
-            csF = valid ? lb1p_child(0, pj, chF, chB) : 0;
-            csB = valid ? lb1p_child(1, pj, chF, chB) : 0;
+            csF = valid ? lb1p_child(0, job, chF, chB) : 0;
+            csB = valid ? lb1p_child(1, job, chF, chB) : 0;
             return;
         }
         uint64_t rowmask = row_jobs(valid, job);
@@ -492,18 +579,19 @@ struct Explorer {
     // -------------------------------------------------------------- leaf
     // ivm_bound::boundLeaf (ivm_bound.cpp:238-271): makespan of front + the
     // last row job + back; the incumbent only moves on strict improvement.
+    __device__ __forceinline__ int makespan_with(int r) const
+    {
+        int base = lane < M ? frow(nf)[lane] : 0;
+        int c = chain_scan(base, pmach(r));
+        int b = lane < M ? brow(nb)[lane] : 0;
+        return wave_max(lane < M ? c + b : 0);
+    }
+
     __device__ __forceinline__ void leaf()
     {
         leaves++;
         const int r = rdlane(rowv, 0);
-        const int pr = pmach(r);
-        int c = 0, cost = 0;
-#pragma unroll
-        for (int q = 0; q < M; q++) {
-            int base = rdlane(F[q], nf);
-            c = (q == 0 ? base : max(base, c)) + rdlane(pr, q);
-            cost = max(cost, c + rdlane(B[q], nb));
-        }
+        const int cost = makespan_with(r);
         if (!gt_best(cost)) {
             if (cost < local_best) {
                 write_perm(r);
@@ -676,7 +764,7 @@ struct Explorer {
     // ivm.cpp:188-237): split at the first line l whose pos differs from end
     // (l <= L, l < n - margin); the thief gets the explorable cells after the
     // victim's kept half, rows 0..l and the victim's end; no re-bounding.
-    // h, t: idle-ring head/tail loaded before the node's work (prefetch)
+    // h, t: idle-ring head/tail loaded before the node's work (prefetch).
     __device__ __forceinline__ void try_donate(unsigned h, unsigned t)
     {
         h = (unsigned)uni((int)h);
@@ -874,9 +962,9 @@ __global__ __launch_bounds__(64) void k_bound_root(KArgs a, const int *root_perm
 
 // ------------------------------------------------- batched bound_abstract
 // Load an explicit partial schedule (perm, l1, l2) into an explorer's
-// register stacks: F <- perm[0..l1] (scheduleFront), B <- perm[l2..n-1]
-// in reverse (scheduleBack), remain <- the middle.  Then the explorer's own
-// routines produce the bounds.
+// stacks: F <- perm[0..l1] (scheduleFront), B <- perm[l2..n-1] in reverse
+// (scheduleBack), remain <- the middle.  Then the explorer's own routines
+// produce the bounds.  Requires (l1+1) + (n-l2) < n.
 template <int M>
 __device__ __forceinline__ void load_partial(Explorer<M, true> &X, const int *perm, int l1, int l2)
 {
@@ -886,21 +974,12 @@ __device__ __forceinline__ void load_partial(Explorer<M, true> &X, const int *pe
 }
 
 // makespan of perm with perm[mid] the only job between the stacks
-// (front = perm[0..mid-1], back = perm[mid+1..n-1]); never pushes an n-th
-// stack entry, so n = 64 keeps every stack index inside the wavefront.
+// (front = perm[0..mid-1], back = perm[mid+1..n-1])
 template <int M>
 __device__ __forceinline__ int makespan_mid(Explorer<M, true> &X, const int *perm, int mid)
 {
     load_partial<M>(X, perm, mid - 1, mid + 1);
-    const int pr = X.pmach(uni(perm[mid]));
-    int c = 0, cost = 0;
-#pragma unroll
-    for (int q = 0; q < M; q++) {
-        int base = rdlane(X.F[q], X.nf);
-        c = (q == 0 ? base : max(base, c)) + rdlane(pr, q);
-        cost = max(cost, c + rdlane(X.B[q], X.nb));
-    }
-    return cost;
+    return X.makespan_with(uni(perm[mid]));
 }
 
 template <int M>
@@ -925,6 +1004,14 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
         l2 = uni(l2s[s]);
         bad = bad || l1 < -1 || l2 > n || l1 >= l2;
     }
+    if (!bad) {
+        if (lane < n) lds.scr[lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        if (lane < n) atomicAdd(&lds.scr[pv], 1);
+        __builtin_amdgcn_wave_barrier();
+        bad = ballot(lane < n && lds.scr[lane] != 1) != 0ull;  // repeated job ids
+        __builtin_amdgcn_wave_barrier();
+    }
     if (bad) {
         if (which == 0) {
             if (lane < n) o1[(size_t)s * n + lane] = -1;
@@ -941,8 +1028,16 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
         return;
     }
     const int k = l2 - l1 - 1;
-    if (which == 2 && k <= 1) {
-        // bound_fsp_strong.cpp:543-546: at most one unscheduled job -> makespan
+    if (which == 0 && k == 0) {
+        if (lane < n) {
+            o1[(size_t)s * n + lane] = 0;
+            if (arg) o2[(size_t)s * n + lane] = 0;
+        }
+        return;
+    }
+    if ((which == 1 && k == 0) || (which == 2 && k <= 1)) {
+        // nothing (LB1') or at most one job (LB2, bound_fsp_strong.cpp:543-546)
+        // unscheduled: both reduce to the makespan of the completed permutation
         int v = makespan_mid<M>(X, perm, k == 1 ? l1 + 1 : (l1 >= 0 ? l1 : 0));
         if (lane == 0) o1[s] = v;
         return;
@@ -952,18 +1047,17 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
     const int job = valid ? perm[l1 + 1 + lane] : 0;
     if (which == 0) {
         // bound_fsp_weak::boundChildren (bound_fsp_weak.cpp:193-221)
-        int pj[M];
-#pragma unroll
-        for (int q = 0; q < M; q++) pj[q] = lds.ptmT[job * X.mp + q];
-        int lbF = X.lb1_front(pj, nullptr);
-        int lbB = arg ? X.lb1_back(pj, nullptr) : 0;
+        X.rowv = job;
+        X.L = n - k;
+        int cb, ce;
+        X.lb1_children(k, cb, ce, nullptr);
         int *inv = lds.scr;
         if (lane < n) inv[pv] = lane;
         __builtin_amdgcn_wave_barrier();
         int ci = lane < n ? inv[lane] - (l1 + 1) : -1;
         bool child = ci >= 0 && ci < k;
-        int vF = shfl(lbF, child ? ci : 0);
-        int vB = shfl(lbB, child ? ci : 0);
+        int vF = shfl(cb, child ? ci : 0);
+        int vB = shfl(ce, child ? ci : 0);
         if (lane < n) {
             o1[(size_t)s * n + lane] = child ? vF : 0;
             if (arg) o2[(size_t)s * n + lane] = child ? vB : 0;
@@ -973,7 +1067,7 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
         int t0 = 0, lb = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
-            int f = rdlane(X.F[q], X.nf), b = rdlane(X.B[q], X.nb), r = rdlane(X.remv, q);
+            int f = X.fval(q), b = X.bval(q), r = X.rval(q);
             if (q == 0) {
                 t0 = f + r;
                 lb = t0 + b;
@@ -988,18 +1082,9 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
         // bound_fsp_strong::bornes_calculer (bound_fsp_strong.cpp:540-560)
         uint64_t um = X.row_jobs(valid, job);
         int *fs = lds.scr, *bs = lds.scr + 32;
-        int fl = 0, bl = 0;
-#pragma unroll
-        for (int q = 0; q < M; q++) {
-            int fq = rdlane(X.F[q], X.nf), bq = rdlane(X.B[q], X.nb);
-            if (lane == q) {
-                fl = fq;
-                bl = bq;
-            }
-        }
         if (lane < M) {
-            fs[lane] = fl;
-            bs[lane] = bl;
+            fs[lane] = X.frow(X.nf)[lane];
+            bs[lane] = X.brow(X.nb)[lane];
         }
         __builtin_amdgcn_wave_barrier();
         DevInst J = a.I;
@@ -1045,7 +1130,9 @@ hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
 template <int M>
 hipError_t launch_root_m(const KArgs &a, const int *root_perm, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_bound_root<M>, dim3(1), dim3(64), root_lds_bytes(a.I), s, a, root_perm);
+    const size_t lds = root_lds_bytes(a.I);
+    hipFuncSetAttribute((const void *)k_bound_root<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_bound_root<M>, dim3(1), dim3(64), lds, s, a, root_perm);
     return hipGetLastError();
 }
 
@@ -1054,7 +1141,7 @@ hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perm
                            const int *l2, int arg, int *o1, int *o2, hipStream_t s)
 {
     const bool lb2 = which == 2 && jtab_in_lds(a.I.n, a.I.npairs);
-    const size_t lds = lds_table_bytes(a.I.n, a.I.mp, a.I.npairs, lb2) + 4 * lds_wave_bytes(a.I.n);
+    const size_t lds = lds_table_bytes(a.I.n, a.I.mp, a.I.npairs, lb2) + 4 * lds_wave_bytes(a.I.n, a.I.m);
     hipFuncSetAttribute((const void *)k_bounds<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_bounds<M>, dim3((count + 3) / 4), dim3(256), lds, s, a, which, count, perms, l1, l2,
                        arg, o1, o2);
