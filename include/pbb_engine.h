@@ -31,6 +31,7 @@ extern "C" {
 #define PBB_E_STATE -3       /* call not valid in the engine's current state */
 #define PBB_E_CAPACITY -4    /* buffer / device capacity exceeded */
 #define PBB_E_NODEVICE -5    /* no usable gfx950 device */
+#define PBB_E_BUSY -6        /* serving: no free slot or HBM budget for this request now */
 
 #define PBB_MAX_JOBS 63      /* n <= 63: IVM row cells and stack entries 0..n fit one 64-lane wavefront */
 #define PBB_MAX_MACHINES 20  /* m <= 20 (the reference GPU cap, gpu_fsp_bound.cuh:12) */
@@ -185,6 +186,89 @@ int pbb_interrupt(pbb_engine *e);
 
 /* Explorer capacity the engine would use for (n, m, cfg) on this device. */
 int pbb_query_explorers(int n, int m, const pbb_config *cfg, int *explorers);
+
+/* --------------------------------------------- multi-instance pool
+ * Many instances in one resident launch: batched solves (all instances in
+ * one HIP-graph launch) and a serving loop (requests admitted while others
+ * run).  There is no reference counterpart: the reference solves one
+ * instance per process (its strong-bound tables are process-global,
+ * bounds/src/bound_fsp_strong.cpp:5-10); each instance is solved with the
+ * same semantics as pbb_create + pbb_init_full + pbb_run with the root order
+ * the identity (initial.sol=0).  Instances with m < machines are padded
+ * with zero-time machines, which leaves every bound and makespan unchanged. */
+typedef struct pbb_pool pbb_pool;
+
+#define PBB_RES_PENDING 0
+#define PBB_RES_SOLVED 1     /* explored to the end: best is the optimum (or no schedule beats ub) */
+#define PBB_RES_ABORTED 2    /* time budget or interrupt before the end */
+#define PBB_RES_INVALID 3    /* malformed request */
+
+typedef struct pbb_pool_params {
+    int max_jobs;               /* largest n accepted (3..63) */
+    int machines;               /* M: instances with 2 <= m <= M are accepted */
+    int slots;                  /* instance slots (instances in flight) */
+    int waves_per_group;        /* explorers per workgroup: 1, 2 or 4 */
+    int explorers_per_instance; /* cap on the explorers working one instance */
+    int steal_margin;           /* split only at lines < n - margin; 0 = the config's */
+    int groups;                 /* workgroups launched; 0 = every resident one */
+    double hbm_budget_gb;       /* serving admission budget; 0 = device memory free at creation */
+} pbb_pool_params;
+
+typedef struct pbb_result {
+    int tag;                    /* caller's tag */
+    int status;                 /* PBB_RES_* */
+    int best;                   /* optimum, or ub when nothing below ub exists */
+    int improved;               /* 1: perm holds a schedule of cost best */
+    uint64_t decomposed, leaves, simple_bounds, johnson_bounds;
+    double admit_ms, done_ms;   /* device clock, ms after the launch started */
+    int perm[PBB_MAX_JOBS];
+} pbb_result;
+
+int pbb_pool_params_default(pbb_pool_params *prm);
+int pbb_pool_create(const pbb_config *cfg, const pbb_pool_params *prm, pbb_pool **out);
+int pbb_pool_destroy(pbb_pool *p);
+/* Explorers and workgroups the pool launches. */
+int pbb_pool_info(pbb_pool *p, int *explorers, int *groups, int *lds_bytes);
+
+/* Batch: count <= slots instances, instance i's m[i] x n[i] machine-major
+ * times at ptms[off_i], off_i = sum of n[k]*m[k] for k < i.  Upload, reset,
+ * the resident launch and the result download are one graph launch.
+ * tags may be NULL (tag = index). kernel_ms (may be NULL) = graph time. */
+int pbb_pool_solve(pbb_pool *p, int count, const int32_t *ptms, const int *n, const int *m,
+                   const int *ub, const int *tags, double time_budget_s, pbb_result *out,
+                   double *kernel_ms);
+
+/* Serving: start the resident launch (ends after time_budget_s at most). */
+int pbb_serve_start(pbb_pool *p, double time_budget_s);
+/* Admit one request if a slot and the HBM budget allow, else PBB_E_BUSY. */
+int pbb_serve_submit(pbb_pool *p, const int32_t *ptm, int n, int m, int ub, int tag);
+/* Completed requests since the last poll (at most max). */
+int pbb_serve_poll(pbb_pool *p, pbb_result *out, int max, int *count);
+/* Close the queue, let admitted requests finish, wait for the launch. */
+int pbb_serve_stop(pbb_pool *p);
+/* Admission footprint of an n-job request: explorers_per_instance x
+ * (n*n + 2n) int32 (the IVM of one explorer) plus the slot's tables. */
+int pbb_serve_footprint(pbb_pool *p, int n, uint64_t *bytes);
+
+/* Replay an arrival stream through the serving loop: request i (as in
+ * pbb_pool_solve) arrives at arrival_s[i] seconds; latency_ms[i] = completion
+ * seen by the host - arrival.  *wall_s = first arrival to last completion. */
+int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, const int *n, const int *m,
+                     const int *ub, const double *arrival_s, double time_budget_s, pbb_result *out,
+                     double *latency_ms, double *wall_s);
+/* The same stream on the one-instance engine, one launch per instance in
+ * arrival order (first come, first served). */
+int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, const int *n,
+                     const int *m, const int *ub, const double *arrival_s, double time_budget_s,
+                     pbb_result *out, double *latency_ms, double *wall_s);
+
+/* Autotuner: time pbb_pool_solve on the given instances for every
+ * (waves_per_group, explorers_per_instance, steal_margin) candidate and
+ * write the fastest into *best (other fields taken from *base).  trials
+ * (may be NULL) receives up to max_trials rows {W, epi, margin, ms}. */
+int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *base, int count,
+                      const int32_t *ptms, const int *n, const int *m, const int *ub,
+                      pbb_pool_params *best, double *trials, int max_trials, int *ntrials);
 
 const char *pbb_last_error(void);
 const char *pbb_version(void);

@@ -15,7 +15,10 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "pbb_engine.h")
 PBB_MAX_JOBS = 63
 PBB_MAX_MACHINES = 20
 
-ERRORS = {-1: "PBB_E_ARG", -2: "PBB_E_HIP", -3: "PBB_E_STATE", -4: "PBB_E_CAPACITY", -5: "PBB_E_NODEVICE"}
+ERRORS = {-1: "PBB_E_ARG", -2: "PBB_E_HIP", -3: "PBB_E_STATE", -4: "PBB_E_CAPACITY", -5: "PBB_E_NODEVICE",
+          -6: "PBB_E_BUSY"}
+PBB_E_BUSY = -6
+RES_PENDING, RES_SOLVED, RES_ABORTED, RES_INVALID = 0, 1, 2, 3
 
 
 class PbbError(RuntimeError):
@@ -44,6 +47,30 @@ class Stats(C.Structure):
         ("decomposed", C.c_uint64), ("leaves", C.c_uint64), ("simple_bounds", C.c_uint64),
         ("johnson_bounds", C.c_uint64), ("explorers", C.c_int), ("active", C.c_int),
         ("kernel_ms", C.c_double), ("depth_histo", C.c_uint64 * (PBB_MAX_JOBS + 2)),
+    ]
+
+
+class PoolParams(C.Structure):
+    """pbb_pool_params: shape and tuning of a multi-instance pool."""
+
+    _fields_ = [
+        ("max_jobs", C.c_int), ("machines", C.c_int), ("slots", C.c_int),
+        ("waves_per_group", C.c_int), ("explorers_per_instance", C.c_int),
+        ("steal_margin", C.c_int), ("groups", C.c_int), ("hbm_budget_gb", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Result(C.Structure):
+    """pbb_result: one solved (or aborted) instance of a pool."""
+
+    _fields_ = [
+        ("tag", C.c_int), ("status", C.c_int), ("best", C.c_int), ("improved", C.c_int),
+        ("decomposed", C.c_uint64), ("leaves", C.c_uint64), ("simple_bounds", C.c_uint64),
+        ("johnson_bounds", C.c_uint64), ("admit_ms", C.c_double), ("done_ms", C.c_double),
+        ("perm", C.c_int * PBB_MAX_JOBS),
     ]
 
 
@@ -80,6 +107,24 @@ SIGNATURES = {
     "pbb_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
     "pbb_interrupt": (C.c_int, [VP]),
     "pbb_query_explorers": (C.c_int, [C.c_int, C.c_int, C.POINTER(Config), IP]),
+    "pbb_pool_params_default": (C.c_int, [C.POINTER(PoolParams)]),
+    "pbb_pool_create": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.POINTER(VP)]),
+    "pbb_pool_destroy": (C.c_int, [VP]),
+    "pbb_pool_info": (C.c_int, [VP, IP, IP, IP]),
+    "pbb_pool_solve": (C.c_int, [VP, C.c_int, I32P, IP, IP, IP, IP, C.c_double, C.POINTER(Result),
+                                 C.POINTER(C.c_double)]),
+    "pbb_serve_start": (C.c_int, [VP, C.c_double]),
+    "pbb_serve_submit": (C.c_int, [VP, I32P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "pbb_serve_poll": (C.c_int, [VP, C.POINTER(Result), C.c_int, IP]),
+    "pbb_serve_stop": (C.c_int, [VP]),
+    "pbb_serve_footprint": (C.c_int, [VP, C.c_int, U64P]),
+    "pbb_serve_replay": (C.c_int, [VP, C.c_int, I32P, IP, IP, IP, C.POINTER(C.c_double), C.c_double,
+                                   C.POINTER(Result), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "pbb_plain_replay": (C.c_int, [C.POINTER(Config), C.c_int, I32P, IP, IP, IP, C.POINTER(C.c_double),
+                                   C.c_double, C.POINTER(Result), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
+    "pbb_pool_autotune": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.c_int, I32P, IP, IP, IP,
+                                    C.POINTER(PoolParams), C.POINTER(C.c_double), C.c_int, IP]),
     "pbb_last_error": (C.c_char_p, []),
     "pbb_version": (C.c_char_p, []),
 }

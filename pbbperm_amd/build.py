@@ -1,7 +1,8 @@
 """Build recipe for the in-tree gfx950 engine library ``libpbb_engine.so``.
 
-The kernel source is compiled once per machine count M (2..20, -DPBB_M=M)
-in parallel, then linked with the runtime-M dispatch and the host C ABI.
+The kernel sources (one-instance engine, multi-instance pool) are compiled
+once per machine count M (2..20, -DPBB_M=M) in parallel, then linked with
+the runtime-M dispatch and the host C ABI.
 The library is built in-tree so it travels with the repository snapshot.
 """
 from __future__ import annotations
@@ -40,9 +41,10 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     jobs = jobs or min(16, os.cpu_count() or 4)
     units = []
     for m in MACHINES:
-        obj = os.path.join(BUILD, f"pbb_kernels_m{m}.o")
-        units.append([HIPCC, *FLAGS, f"-DPBB_M={m}", "-c", os.path.join(CSRC, "pbb_kernels.hip"), "-o", obj])
-    for src in ("pbb_dispatch.cpp", "pbb_host.cpp"):
+        for src in ("pbb_kernels", "pbb_pool"):
+            obj = os.path.join(BUILD, f"{src}_m{m}.o")
+            units.append([HIPCC, *FLAGS, f"-DPBB_M={m}", "-c", os.path.join(CSRC, src + ".hip"), "-o", obj])
+    for src in ("pbb_dispatch.cpp", "pbb_host.cpp", "pbb_pool.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         units.append([HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(jobs) as ex:

@@ -57,4 +57,18 @@ hipError_t launch_bounds(int which, const KArgs &a, int count, const int *perms,
 #undef C_
 }
 
+hipError_t launch_pool(int m, const KArgs &a, const PoolArgs &p, int groups, int init_count, hipStream_t s)
+{
+#define C_(M) launch_pool_m<M>(a, p, groups, init_count, s)
+    PBB_FOR_M(C_, m)
+#undef C_
+}
+
+hipError_t pool_occupancy(int m, const KArgs &a, const PoolArgs &p, int *groups_per_cu)
+{
+#define C_(M) pool_occupancy_m<M>(a, p, groups_per_cu)
+    PBB_FOR_M(C_, m)
+#undef C_
+}
+
 }  // namespace pbb

@@ -1,0 +1,923 @@
+// pbb_explorer.h -- the explorer: one IVM branch-and-bound search per
+// 64-lane wavefront (navigation, bounding, branching, pruning, interval
+// sharing), shared by the one-instance engine kernels (pbb_kernels.hip) and
+// the multi-instance pool kernel (pbb_pool.hip).  Device code only; every
+// function is force-inlined into the kernels that use it.
+#pragma once
+
+#include "pbb_device.h"
+#include "pbb_kernels.h"
+
+namespace pbb {
+
+// ------------------------------------------------------------------ atomics
+__device__ __forceinline__ int ld_acq(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_rlx(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_rlx(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rlx(int *p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// lane-0 read-modify-writes, result broadcast to the wavefront
+__device__ __forceinline__ int w_add(int *p, int v)
+{
+    int r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return rdlane(r, 0);
+}
+__device__ __forceinline__ unsigned w_addu(unsigned *p, unsigned v)
+{
+    unsigned r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    return (unsigned)rdlane((int)r, 0);
+}
+__device__ __forceinline__ unsigned long long w_add64(unsigned long long *p, unsigned long long v)
+{
+    unsigned long long r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned lo = (unsigned)rdlane((int)(unsigned)r, 0);
+    unsigned hi = (unsigned)rdlane((int)(unsigned)(r >> 32), 0);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ bool w_cas(unsigned *p, unsigned expect, unsigned desired)
+{
+    int ok = 0;
+    if (lane_id() == 0) {
+        unsigned e = expect;
+        ok = __hip_atomic_compare_exchange_strong(p, &e, desired, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return rdlane(ok, 0) != 0;
+}
+__device__ __forceinline__ void w_min64(unsigned long long *p, unsigned long long v)
+{
+    if (lane_id() == 0) __hip_atomic_fetch_min(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------ lane scans (DPP)
+// Inclusive prefix scans over lanes 0..31 (row_shr 1,2,4,8 inside each
+// 16-lane row, then row_bcast:15 into row 1).  Lanes only combine with lanes
+// to their left, so values in lanes >= the vector length are never read.
+__device__ __forceinline__ int scan_add32(int x)
+{
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xF, 0xF, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xA, 0xF, false);
+    return x;
+}
+__device__ __forceinline__ int scan_max32(int x)
+{
+    const int lo = (int)0x80000000;
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x111, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x112, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x114, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x118, 0xF, 0xF, false));
+    x = max(x, __builtin_amdgcn_update_dpp(lo, x, 0x142, 0xA, 0xF, false));
+    return x;
+}
+
+// Max-plus chain of one job over the machines, lane q = machine q:
+// c_q = max(base_q, c_{q-1}) + p_q  ==  P_q + max_{i<=q}(base_i - P_{i-1})
+// with P the prefix sum of p (the recurrence of bound_fsp_weak.cpp:76-82).
+__device__ __forceinline__ int chain_scan(int base, int p)
+{
+    const int P = scan_add32(p);
+    return P + scan_max32(base - (P - p));
+}
+
+// ------------------------------------------------------------ LDS layout
+struct Lds {
+    int *ptmT;      // [n][mp] job-major (uniform per-job rows)
+    int *pm;        // [m][n] machine-major (per-lane loads of different jobs: distinct banks)
+    int *minA, *minD, *tot;
+    int4 *jtab;     // Johnson table (LDS or global)
+    int8_t *mat;    // this wave's IVM matrix, row r at mat + r*64
+    int *stk;       // this wave's stacks: F entry s at row s, B entry s at row n-s ((n+1) x m)
+    int2 *U;        // this wave's LB1 operand block [2][m]
+    int *scr;       // this wave's scratch (64 ints)
+};
+
+__device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load_jtab)
+{
+    Lds L;
+    const int n = I.n, mp = I.mp, m = I.m;
+    L.ptmT = (int *)smem;
+    L.pm = L.ptmT + n * mp;
+    L.minA = L.pm + n * mp;
+    L.minD = L.minA + mp;
+    L.tot = L.minD + mp;
+    size_t off = (size_t)(2 * n + 3) * mp * 4;
+    off = (off + 15) & ~(size_t)15;
+    load_jtab = load_jtab && jtab_in_lds(n, I.npairs);
+    L.jtab = load_jtab ? (int4 *)(smem + off) : const_cast<int4 *>(I.jtab);
+    char *wbase = smem + lds_table_bytes(n, mp, I.npairs, load_jtab) +
+                  (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n, m);
+    L.mat = (int8_t *)wbase;
+    L.stk = (int *)(wbase + (size_t)n * 64);
+    L.U = (int2 *)(wbase + (size_t)n * 64 + lds_stack_bytes(n, m));
+    L.scr = (int *)(wbase + (size_t)n * 64 + lds_stack_bytes(n, m) + (size_t)m * 16);
+    for (int i = threadIdx.x; i < n * mp; i += blockDim.x) L.ptmT[i] = I.ptmT[i];
+    for (int i = threadIdx.x; i < n * m; i += blockDim.x) L.pm[i] = I.ptm[i];
+    for (int i = threadIdx.x; i < mp; i += blockDim.x) {
+        L.minA[i] = i < m ? I.min_arr[i] : 0;
+        L.minD[i] = i < m ? I.min_dep[i] : 0;
+        int s = 0;
+        if (i < m)
+            for (int j = 0; j < n; j++) s += I.ptmT[j * mp + i];
+        L.tot[i] = s;
+    }
+    if (load_jtab)
+        for (int i = threadIdx.x; i < I.npairs * n; i += blockDim.x) L.jtab[i] = I.jtab[i];
+    __syncthreads();
+    return L;
+}
+
+// instance reference of the one-instance engine: everything from the
+// kernel arguments
+__device__ __forceinline__ InstRef engine_inst(const KArgs &a)
+{
+    InstRef r;
+    r.ctl = a.ctl;
+    r.ring_seq = a.ring_seq;
+    r.ring_id = a.ring_id;
+    r.ring_cap = a.ring_cap;
+    r.ub0 = a.initial_ub;
+    r.best = a.best;
+    r.root_row = a.root_row;
+    r.root_dir = a.root_dir;
+    r.m = a.I.m;
+    r.npairs = a.I.npairs;
+    r.pairs_mode = a.pairs_mode;
+    r.pair_ab = a.I.pair_ab;
+    return r;
+}
+
+// ------------------------------------------------------------- explorer
+// Stacks: F entry s (row s of stk) = completion times on every machine of
+// the first s front jobs, F entry 0 = min_arr (the front of an empty prefix,
+// bound_fsp_weak.cpp:67-70); B entry s (row n-s) = reverse completion times
+// of the last s back jobs, B entry 0 = min_dep.  nf + nb < n, so the two
+// never overlap.  A chain started from min_arr/min_dep equals one started
+// from zero, since min_arr[q] never exceeds a single job's head (and min_dep
+// its tail).
+template <int M, bool REF>
+struct Explorer {
+    const KArgs &a;
+    Lds lds;
+    InstRef ir;   // the instance this explorer works on
+    Ctl *run;     // stop / error words of the launch
+    int n, mp, lane, e;
+    int rows;     // row count of the HBM explorer state (>= n)
+    int L;                            // current line
+    int rowv, posv, endv, dirv, remv; // lane vectors (remv: lane q = remaining work on machine q)
+    int nf, nb;
+    int local_best;
+    unsigned dec, leaves, sb, jb;     // deltas since the last flush
+    int hist;
+    unsigned long long t_end;
+
+    __device__ __forceinline__ Explorer(const KArgs &args, const Lds &l, int eid, const InstRef &r, Ctl *runctl,
+                                        int n_, int mp_)
+        : a(args), lds(l), ir(r), run(runctl), n(n_), mp(mp_), lane(lane_id()), e(eid), rows(args.rows)
+    {
+        L = 0;
+        rowv = posv = endv = dirv = remv = 0;
+        nf = nb = 0;
+        dec = leaves = sb = jb = 0;
+        hist = 0;
+        local_best = PBB_INF;
+        t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
+    }
+
+    __device__ __forceinline__ bool gt_best(int v) const
+    {
+        return a.find_all ? v > local_best : v >= local_best;
+    }
+
+    __device__ __forceinline__ int *frow(int s) const { return lds.stk + s * M; }
+    __device__ __forceinline__ int *brow(int s) const { return lds.stk + (n - s) * M; }
+    // uniform stack values (LDS broadcast reads)
+    __device__ __forceinline__ int fval(int q) const { return frow(nf)[q]; }
+    __device__ __forceinline__ int bval(int q) const { return brow(nb)[q]; }
+    __device__ __forceinline__ int rval(int q) const { return rdlane(remv, q); }
+
+    __device__ __forceinline__ int pmach(int job) const  // lane q: p[q][job]
+    {
+        return lane < M ? lds.ptmT[job * mp + lane] : 0;
+    }
+
+    __device__ __forceinline__ void reset_stacks()
+    {
+        nf = nb = 0;
+        if (lane < M) {
+            frow(0)[lane] = lds.minA[lane];
+            brow(0)[lane] = lds.minD[lane];
+        }
+        remv = lane < M ? lds.tot[lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // schedule job j in direction d: push a new F (d=0) or B (d=1) entry,
+    // computed lane-parallel by chain_scan; remain -= p_j.
+    __device__ __forceinline__ void push_job(int j, int d)
+    {
+        const int pk = pmach(j);
+        if (d == 0) {
+            int base = lane < M ? frow(nf)[lane] : 0;
+            int c = chain_scan(base, pk);
+            if (lane < M) frow(nf + 1)[lane] = c;
+            nf++;
+        } else {
+            // back chain runs from machine M-1 down: lane q handles machine M-1-q
+            int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
+            int pr = lane < M ? lds.ptmT[j * mp + (M - 1 - lane)] : 0;
+            int c = chain_scan(base, pr);
+            if (lane < M) brow(nb + 1)[M - 1 - lane] = c;
+            nb++;
+        }
+        remv -= pk;
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    __device__ __forceinline__ void pop_job(int j, int d)
+    {
+        if (d == 0)
+            nf--;
+        else
+            nb--;
+        remv += pmach(j);
+    }
+
+    // ivm::beforeEnd (ivm.cpp:152-160): lexicographic pos <= end
+    __device__ __forceinline__ bool before_end() const
+    {
+        bool in = lane < n;
+        uint64_t lt = ballot(in && posv < endv);
+        uint64_t gt = ballot(in && posv > endv);
+        uint64_t d = lt | gt;
+        if (d == 0) return true;
+        int f = __ffsll((unsigned long long)d) - 1;
+        return (lt >> f) & 1ull;
+    }
+
+    // ------------------------------------------------------- LB1 children
+    // bound_fsp_weak::addFrontAndBound / addBackAndBound (bound_fsp_weak.cpp:133-191)
+    // for all children of the node, both directions in one pass: lane i < 32
+    // bounds child base+i in the front direction, lane 32+i the same child in
+    // the back direction.  The back recurrence run over machines M-1..0 has
+    // the same form as the front one run over 0..M-1, so both halves execute
+    // one loop:  t1 = max(x_q, t0); lb = max(lb, t1 + y_q); t0 = t1 + p_q,
+    // with (x_q, y_q) = (front_q, remain_q + back_q) in the front half and
+    // (back_{M-1-q}, front_{M-1-q} + remain_{M-1-q}) in the back half, read
+    // from the per-wave operand block as LDS broadcasts.
+    __device__ __forceinline__ void build_operands() const
+    {
+        if (lane < M) {
+            const int f = frow(nf)[lane], b = brow(nb)[lane], r = remv;
+            lds.U[lane] = make_int2(f, r + b);
+            lds.U[M + (M - 1 - lane)] = make_int2(b, f + r);
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+
+    // one fused pass over children base..base+31; returns this lane's bound
+    // (chain receives t0 after every step when non-null: the child's own
+    // front chain in machine order (front half) or back chain in reversed
+    // machine order (back half))
+    __device__ __forceinline__ int lb1_pass(int base, int k, int *chain) const
+    {
+        const int half = lane >> 5;
+        const int c = base + (lane & 31);
+        const int job = base == 0 ? bcast_low_half(rowv) : bcast_high_half(rowv);
+        const int jb = c < k ? job : 0;
+        // machine-major rows: front lanes walk machines 0..M-1, back lanes M-1..0
+        const int *pcol = lds.pm + (half ? (M - 1) * n : 0) + jb;
+        const int pstep = half ? -n : n;
+        const int2 *u = lds.U + half * M;
+        int pj[M];
+#pragma unroll
+        for (int q = 0; q < M; q++) pj[q] = pcol[q * pstep];
+        int2 o = u[0];
+        int t0 = o.x + pj[0];
+        int lb = o.x + o.y;
+        if (chain) chain[0] = t0;
+#pragma unroll
+        for (int q = 1; q < M; q++) {
+            o = u[q];
+            int t1 = max(o.x, t0);
+            lb = max(lb, t1 + o.y);
+            t0 = t1 + pj[q];
+            if (chain) chain[q] = t0;
+        }
+        return lb;
+    }
+
+    // row layout (lane = child): cb (front) and ce (back) for k <= 64 children
+    __device__ __forceinline__ void lb1_children(int k, int &cb, int &ce, int *chain0) const
+    {
+        build_operands();
+        int lb0 = lb1_pass(0, k, chain0);
+        if (k <= 32) {
+            cb = lb0;
+            ce = bcast_high_half(lb0);
+        } else {
+            int lb1 = lb1_pass(32, k, nullptr);
+            int sh0 = bcast_high_half(lb0), sh1 = bcast_low_half(lb1);
+            cb = lane < 32 ? lb0 : sh1;
+            ce = lane < 32 ? sh0 : lb1;
+        }
+    }
+
+    // chooseChildrenSet (ivm_bound.cpp:413-507): lanes beyond the row hold
+    // the reference's zero entries and are excluded from every term.
+    __device__ __forceinline__ int choose(bool valid, int cb, int ce) const
+    {
+        switch (a.branching) {
+        case -3: return (L % 2) ? 1 : 0;
+        case -2: return 1;
+        case -1: return 0;
+        case 1: {
+            int s = wave_sum(valid ? cb - ce : 0);
+            return s < 0 ? 1 : 0;
+        }
+        case 2: {
+            int ub = a.single_node ? local_best : ir.ub0;
+            int elim = __popcll(ballot(valid && cb >= ub)) - __popcll(ballot(valid && ce >= ub));
+            int s = wave_sum(valid ? ((cb < ub ? cb : 0) - (ce < ub ? ce : 0)) : 0);
+            if (elim > 0) return 0;
+            if (elim < 0) return 1;
+            return s < 0 ? 1 : 0;
+        }
+        default: {
+            int ub = a.single_node ? local_best : ir.ub0;
+            int mn = wave_min(valid ? min(cb > 0 ? cb : PBB_INF, ce > 0 ? ce : PBB_INF) : PBB_INF);
+            int minc = __popcll(ballot(valid && cb == mn)) - __popcll(ballot(valid && ce == mn));
+            int elimc = __popcll(ballot(valid && cb >= ub)) - __popcll(ballot(valid && ce >= ub));
+            if (minc > 0) return 1;
+            if (minc < 0) return 0;
+            return elimc > 0 ? 0 : 1;
+        }
+        }
+    }
+
+    // ----------------------------------------------------------- bounding
+    // Bound the children of the node at line L (row in rowv, unflagged),
+    // choose the direction, refine, prune, reverse (ivm_bound.cpp:141-236,
+    // :337-676).  root: boundRoot always uses weakBoundPrune.
+    __device__ __forceinline__ void bound_row(bool root)
+    {
+        const int k = n - L;
+        const bool valid = lane < k;
+        const int job = valid ? rowv : 0;
+        const bool both = a.branching > 0;
+        int dir, c1, c2 = 0;
+        bool refined = false;
+        if constexpr (!REF) {
+            int cb, ce;
+            lb1_children(k, cb, ce, nullptr);
+            if (!both) ce = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
+            sb += (unsigned)(both ? 2 * k : k);
+            dir = choose(valid, cb, ce);
+            c1 = dir ? ce : cb;
+        } else {
+            const int mode = root ? 0 : a.bound_mode;
+            int cb, ce;
+            lb1_children(k, cb, ce, nullptr);
+            if (mode == 1) {
+                int csF = 0, csB = 0;
+                refine_both(valid, job, csF, csB);
+                dir = choose(valid, csF, csB);
+                c1 = c2 = dir ? csB : csF;
+                refined = true;
+            } else {
+                if (!both) ce = 0;
+                sb += (unsigned)(both ? 2 * k : k);
+                dir = choose(valid, cb, ce);
+                c1 = dir ? ce : cb;
+                if (mode == 2) {
+                    c2 = refine_dir(dir, valid, job, c1);
+                    refined = true;
+                }
+            }
+        }
+        bool pr = valid && (gt_best(c1) || (refined && gt_best(c2)));
+        int v = pr ? flag_neg(job) : job;
+        int prev = L > 0 ? rdlane(dirv, L - 1) : 0;
+        if (prev != dir) v = shfl(v, k - 1 - lane);
+        rowv = valid ? v : 0;
+        dirv = wrlane(dirv, dir, L);
+    }
+
+    // ---------------------------------------------------------- refinement
+    __device__ __forceinline__ uint64_t row_jobs(bool valid, int job) const
+    {
+        uint64_t bits = valid ? (1ull << job) : 0ull;
+        unsigned lo = (unsigned)bits, hi = (unsigned)(bits >> 32);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo |= (unsigned)__shfl_xor((int)lo, o, PBB_WAVE);
+            hi |= (unsigned)__shfl_xor((int)hi, o, PBB_WAVE);
+        }
+        return ((uint64_t)(unsigned)uni((int)hi) << 32) | (unsigned)uni((int)lo);
+    }
+
+    // The refinement bounds below work on one child per lane (lane = row
+    // cell), both directions, with the child's chains computed per lane
+    // (the refinement modes serve instances of up to 30 jobs; LB1 above is
+    // the fused hot loop).
+    // d = 0: front' = front + job; d = 1: back' = job + back.
+    __device__ __forceinline__ void child_chains(int job, int (&chF)[M], int (&chB)[M]) const
+    {
+        const int *pj = lds.ptmT + job * mp;
+        int t = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int f = fval(q);
+            t = (q == 0 ? f : max(f, t)) + pj[q];
+            chF[q] = t;
+        }
+#pragma unroll
+        for (int q = M - 1; q >= 0; q--) {
+            int b = bval(q);
+            t = (q == M - 1 ? b : max(b, t)) + pj[q];
+            chB[q] = t;
+        }
+    }
+
+    // LB1' of this lane's child (bound_fsp_weak::bornes_calculer with the
+    // child's front or back)
+    __device__ __forceinline__ int lb1p_child(int d, int job, const int (&chF)[M], const int (&chB)[M]) const
+    {
+        const int *pj = lds.ptmT + job * mp;
+        int t0 = 0, lb = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int f = d == 0 ? chF[q] : fval(q);
+            int b = d == 1 ? chB[q] : bval(q);
+            int r = rval(q) - pj[q];
+            if (q == 0) {
+                t0 = f + r;
+                lb = t0 + b;
+            } else {
+                int t1 = max(f + r, t0);
+                lb = max(lb, t1 + b);
+                t0 = t1;
+            }
+        }
+        return lb;
+    }
+
+    // Johnson LB2 of child c (uniform), direction d
+    // (bound_fsp_strong::bornes_calculer with the child's partial schedule)
+    __device__ __forceinline__ int lb2_child(int c, int d, const int (&chF)[M], const int (&chB)[M],
+                                             uint64_t rowmask, int k)
+    {
+        const int cj = flag_abs(rdlane(rowv, c));
+        if (k <= 2) {
+            // l2 - l1 <= 2: makespan of the completed permutation (bound_fsp_strong.cpp:543-546)
+            const int other = flag_abs(rdlane(rowv, 1 - c));
+            int fr[M];
+#pragma unroll
+            for (int q = 0; q < M; q++) fr[q] = d == 0 ? rdlane(chF[q], c) : fval(q);
+            int po = pmach(other);
+            int cc = 0, best = 0;
+#pragma unroll
+            for (int q = 0; q < M; q++) {
+                cc = (q == 0 ? fr[0] : max(fr[q], cc)) + rdlane(po, q);
+                int bk = d == 1 ? rdlane(chB[q], c) : bval(q);
+                best = max(best, cc + bk);
+            }
+            return best;
+        }
+        int *fs = lds.scr, *bs = lds.scr + 32;
+        int fl = 0, bl = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) {
+            int fq = d == 0 ? rdlane(chF[q], c) : fval(q);
+            int bq = d == 1 ? rdlane(chB[q], c) : bval(q);
+            if (lane == q) {
+                fl = fq;
+                bl = bq;
+            }
+        }
+        if (lane < M) {
+            fs[lane] = fl;
+            bs[lane] = bl;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint64_t unsched = rowmask & ~(1ull << cj);
+        DevInst J{};
+        J.n = n;
+        J.m = ir.m;
+        J.npairs = ir.npairs;
+        J.pair_ab = ir.pair_ab;
+        J.jtab = lds.jtab;
+        int r = lb2_pairs_wave(J, fs, bs, unsched, ir.pairs_mode);
+        __builtin_amdgcn_wave_barrier();
+        return r;
+    }
+
+    // mixed mode (computeStrongBounds, ivm_bound.cpp:141-195): refine the
+    // children not pruned by LB1 in the chosen direction; 0 for the others
+    __device__ __forceinline__ int refine_dir(int d, bool valid, int job, int cw)
+    {
+        const int k = n - L;
+        bool todo = valid && !gt_best(cw);
+        uint64_t tmask = ballot(todo);
+        jb += (unsigned)__popcll(tmask);
+        int chF[M], chB[M];
+        child_chains(job, chF, chB);
+        if (a.refine == 0) {
+            int v = lb1p_child(d, job, chF, chB);
+            return todo ? v : 0;
+        }
+        uint64_t rowmask = row_jobs(valid, job);
+        int cs = 0;
+        uint64_t m = tmask;
+        while (m) {
+            int c = __ffsll((unsigned long long)m) - 1;
+            m &= m - 1;
+            int v = lb2_child(c, d, chF, chB, rowmask, k);
+            if (lane == c) cs = v;
+        }
+        return cs;
+    }
+
+    // strong-only mode (strongBoundPrune, ivm_bound.cpp:668-676)
+    __device__ __forceinline__ void refine_both(bool valid, int job, int &csF, int &csB)
+    {
+        const int k = n - L;
+        jb += 2u * (unsigned)k;
+        int chF[M], chB[M];
+        child_chains(job, chF, chB);
+        if (a.refine == 0) {
+            csF = valid ? lb1p_child(0, job, chF, chB) : 0;
+            csB = valid ? lb1p_child(1, job, chF, chB) : 0;
+            return;
+        }
+        uint64_t rowmask = row_jobs(valid, job);
+        csF = csB = 0;
+        for (int c = 0; c < k; c++) {
+            int v0 = lb2_child(c, 0, chF, chB, rowmask, k);
+            int v1 = lb2_child(c, 1, chF, chB, rowmask, k);
+            if (lane == c) {
+                csF = v0;
+                csB = v1;
+            }
+        }
+    }
+
+    // -------------------------------------------------------------- leaf
+    // ivm_bound::boundLeaf (ivm_bound.cpp:238-271): makespan of front + the
+    // last row job + back; the incumbent only moves on strict improvement.
+    __device__ __forceinline__ int makespan_with(int r) const
+    {
+        int base = lane < M ? frow(nf)[lane] : 0;
+        int c = chain_scan(base, pmach(r));
+        int b = lane < M ? brow(nb)[lane] : 0;
+        return wave_max(lane < M ? c + b : 0);
+    }
+
+    __device__ __forceinline__ void leaf()
+    {
+        leaves++;
+        const int r = rdlane(rowv, 0);
+        const int cost = makespan_with(r);
+        if (!gt_best(cost)) {
+            if (cost < local_best) {
+                write_perm(r);
+                w_min64(ir.best, ((unsigned long long)(unsigned)cost << 32) | (unsigned)e);
+            }
+            local_best = cost;
+        }
+        rowv = wrlane(rowv, flag_neg(r), 0);
+    }
+
+    // decode the leaf schedule (ivm_bound::prepareSchedule) into best_perm[e]
+    __device__ __forceinline__ void write_perm(int mid)
+    {
+        int jl = 0;
+        if (lane < L) jl = flag_abs((int)lds.mat[lane * 64 + posv]);
+        uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        uint64_t fr = ballot(lane < L && dirv == 0);
+        uint64_t bk = ballot(lane < L && dirv != 0);
+        int *dst = a.best_perm + (size_t)e * 64;
+        if (lane < L) {
+            int p = dirv == 0 ? __popcll(fr & below) : n - 1 - __popcll(bk & below);
+            dst[p] = jl;
+        }
+        if (lane == 0) dst[__popcll(fr)] = mid;
+    }
+
+    // ------------------------------------------------------- navigation
+    __device__ __forceinline__ void store_row(int line) const
+    {
+        if (lane < n - line) lds.mat[line * 64 + lane] = (int8_t)rowv;
+    }
+
+    __device__ __forceinline__ void go_up()
+    {
+        if (L > 0) {
+            posv = wrlane(posv, 0, L);
+            L--;
+            rowv = lane < n - L ? (int)lds.mat[L * 64 + lane] : 0;
+            const int p = rdlane(posv, L);
+            pop_job(flag_abs(rdlane(rowv, p)), rdlane(dirv, L));
+            if (lane == p) rowv = flag_neg(rowv);
+        } else {
+            posv = wrlane(posv, rdlane(posv, 0) + 1, 0);
+        }
+    }
+
+    // branch on cell c of line L: generate line L+1 (ivm::goDown/generateLine)
+    __device__ __forceinline__ void go_down(int c, bool explore)
+    {
+        store_row(L);
+        push_job(flag_abs(rdlane(rowv, c)), rdlane(dirv, L));
+        int v = shfl(rowv, lane + (lane >= c ? 1 : 0));
+        L++;
+        rowv = lane < n - L ? flag_abs(v) : 0;
+        if (explore) {
+            posv = wrlane(posv, 0, L);
+            dirv = wrlane(dirv, 0, L);
+        }
+    }
+
+    // sequentialbb::unfold (sequentialbb.cpp:206-264): bounded, uncounted
+    __device__ __forceinline__ void unfold()
+    {
+        while (L < n - 2) {
+            int p = rdlane(posv, L);
+            if (rdlane(rowv, p) < 0) {
+                if (lane > L) posv = 0;
+                break;
+            }
+            go_down(p, false);
+            bound_row(false);
+        }
+    }
+
+    // One call of sequentialbb::next: returns 1 when an interior node was
+    // decomposed and bounded, 0 when the interval is exhausted.
+    __device__ __forceinline__ int step()
+    {
+        while (true) {
+            if (!before_end()) return 0;
+            const int p = rdlane(posv, L);
+            if (p >= n - L) {
+                go_up();
+                continue;
+            }
+            uint64_t live = ballot(lane < n - L && rowv >= 0);
+            uint64_t ahead = live & (~0ull << p);
+            if (!((ahead >> p) & 1ull)) {
+                // goRight across pruned / visited cells (ivm::goRight)
+                int c = ahead ? __ffsll((unsigned long long)ahead) - 1 : n - L;
+                posv = wrlane(posv, c, L);
+                continue;
+            }
+            dec++;
+            go_down(p, true);
+            hist += (lane == L);
+            if (L == n - 1) {
+                leaf();
+                continue;
+            }
+            bound_row(false);
+            return 1;
+        }
+    }
+
+    // ------------------------------------------------------ state I/O
+    __device__ __forceinline__ void rebuild_stacks()
+    {
+        reset_stacks();
+        for (int l = 0; l < L; l++) {
+            int p = rdlane(posv, l);
+            int j = uni(flag_abs((int)lds.mat[l * 64 + p]));
+            push_job(j, rdlane(dirv, l));
+        }
+    }
+
+    __device__ __forceinline__ void load_state()
+    {
+        L = uni(a.line[e]);
+        const int8_t *gm = a.mat + (size_t)e * rows * 64;
+        for (int r = 0; r < L; r++)
+            if (lane < n) lds.mat[r * 64 + lane] = gm[r * 64 + lane];
+        rowv = lane < n - L ? (int)gm[L * 64 + lane] : 0;
+        posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
+        endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
+        dirv = lane < n ? (int)a.dir[(size_t)e * 64 + lane] : 0;
+        __builtin_amdgcn_wave_barrier();
+        rebuild_stacks();
+    }
+
+    __device__ __forceinline__ void save_state(int st)
+    {
+        int8_t *gm = a.mat + (size_t)e * rows * 64;
+        for (int r = 0; r < L; r++)
+            if (lane < n) gm[r * 64 + lane] = lds.mat[r * 64 + lane];
+        if (lane < n) gm[L * 64 + lane] = (int8_t)rowv;
+        if (lane < n) {
+            a.pos[(size_t)e * 64 + lane] = (int8_t)posv;
+            a.end[(size_t)e * 64 + lane] = (int8_t)endv;
+            a.dir[(size_t)e * 64 + lane] = (int8_t)dirv;
+        }
+        if (lane == 0) a.line[e] = L;
+        if (lane == 0) st_rlx(a.state + e, st);
+    }
+
+    // fresh interval: row 0 = bounded root row, pos/end from HBM
+    __device__ __forceinline__ bool start_unfold()
+    {
+        L = 0;
+        rowv = lane < n ? ir.root_row[lane] : 0;
+        posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
+        endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
+        dirv = lane == 0 ? ir.root_dir[0] : 0;
+        reset_stacks();
+        if (!before_end()) return false;
+        unfold();
+        return true;
+    }
+
+    // the whole search space [0...0, (n-1)...0] from the bounded root row
+    // already in rowv / dirv lane 0 (sequentialbb::initFullInterval +
+    // initAtInterval, sequentialbb.cpp:66-108)
+    __device__ __forceinline__ bool start_root()
+    {
+        L = 0;
+        posv = 0;
+        endv = lane < n ? n - 1 - lane : 0;
+        dirv = lane == 0 ? dirv : 0;
+        reset_stacks();
+        if (!before_end()) return false;
+        unfold();
+        return true;
+    }
+
+    // --------------------------------------------------- work sharing
+    __device__ __forceinline__ void fail(int code)
+    {
+        if (lane == 0) {
+            st_rlx(&run->err, code);
+            st_rlx(&run->stop, 1);
+        }
+    }
+
+    // bbthread::shareWork + ivm::cuttingPosition (bbthread.cpp:39-72,
+    // ivm.cpp:188-237): split at the first line l whose pos differs from end
+    // (l <= L, l < n - margin); the thief gets the explorable cells after the
+    // victim's kept half, rows 0..l and the victim's end; no re-bounding.
+    // h, t: idle-ring head/tail loaded before the node's work (prefetch).
+    __device__ __forceinline__ void try_donate(unsigned h, unsigned t)
+    {
+        h = (unsigned)uni((int)h);
+        t = (unsigned)uni((int)t);
+        if (h == t) return;
+        int lim = min(L, n - a.steal_margin);
+        if (lim < 0) lim = 0;
+        uint64_t ne = ballot(lane < n && posv != endv);
+        uint64_t limmask = lim >= 64 ? ~0ull : ((1ull << lim) - 1ull);
+        uint64_t cand = ne & limmask;
+        int l = cand ? __ffsll((unsigned long long)cand) - 1 : lim;
+        int pl = rdlane(posv, l), el = rdlane(endv, l);
+        if (!(pl < el)) return;
+        int rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[l * 64 + lane] : -1);
+        uint64_t expl = ballot(lane > pl && lane <= el && rowl >= 0);
+        int nexp = __popcll(expl);
+        if (nexp < 1) return;
+        if (!w_cas(&ir.ctl->idle_head, h, h + 1)) return;
+        // claim the idle explorer registered at ticket h (bounded MPMC ring cell)
+        const unsigned slot = h & (unsigned)(ir.ring_cap - 1);
+        bool ok = false;
+        for (int spin = 0; spin < PBB_SPIN_LIMIT; spin++) {
+            if ((unsigned)uni(ld_acq((const int *)(ir.ring_seq + slot))) == h + 1u) {
+                ok = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!ok) {
+            fail(PBB_DEV_ERR_RING);
+            return;
+        }
+        const int thief = uni(ld_rlx(ir.ring_id + slot)) - 1;
+        if (lane == 0) st_rel((int *)(ir.ring_seq + slot), (int)(h + (unsigned)ir.ring_cap));
+        if (thief < 0 || thief >= a.E) {
+            fail(PBB_DEV_ERR_RING);
+            return;
+        }
+        int keep = nexp / 2;
+        int cut;
+        if (keep == 0) {
+            cut = pl + 1;
+        } else {
+            uint64_t mm = expl;
+            for (int i = 1; i < keep; i++) mm &= mm - 1;
+            cut = __ffsll((unsigned long long)mm);  // index of the keep-th cell + 1
+        }
+        int8_t *gm = a.mat + (size_t)thief * rows * 64;
+        for (int r = 0; r < l; r++)
+            if (lane < n) gm[r * 64 + lane] = lds.mat[r * 64 + lane];
+        if (lane < n) gm[l * 64 + lane] = (int8_t)rowl;
+        if (lane < n) {
+            a.pos[(size_t)thief * 64 + lane] = (int8_t)(lane < l ? posv : (lane == l ? cut : 0));
+            a.end[(size_t)thief * 64 + lane] = (int8_t)endv;
+            a.dir[(size_t)thief * 64 + lane] = (int8_t)(lane <= l ? dirv : 0);
+        }
+        if (lane == 0) a.line[thief] = l;
+        // the victim keeps [pos, cut-1] at line l and everything below
+        if (lane == l) endv = cut - 1;
+        if (lane > l && lane < n) endv = n - 1 - lane;
+        w_add(&ir.ctl->work_count, 1);
+        if (lane == 0) st_rel(a.state + thief, ST_ACTIVE);
+    }
+
+    // enter the idle ring (cell sequence protocol of a bounded MPMC queue)
+    __device__ __forceinline__ void register_idle()
+    {
+        if (lane == 0) st_rlx(a.state + e, ST_IDLE);
+        unsigned t = w_addu(&ir.ctl->idle_tail, 1u);
+        const unsigned slot = t & (unsigned)(ir.ring_cap - 1);
+        bool ok = false;
+        for (int spin = 0; spin < PBB_SPIN_LIMIT; spin++) {
+            if ((unsigned)uni(ld_acq((const int *)(ir.ring_seq + slot))) == t) {
+                ok = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!ok) {
+            fail(PBB_DEV_ERR_RING);
+        } else {
+            if (lane == 0) st_rlx(ir.ring_id + slot, e + 1);
+            if (lane == 0) st_rel((int *)(ir.ring_seq + slot), (int)(t + 1u));
+        }
+    }
+
+    // give up an exhausted interval; returns true for the explorer whose
+    // release emptied the instance (all explorers idle: it is solved)
+    __device__ __forceinline__ bool go_idle()
+    {
+        register_idle();
+        int left = w_add(&ir.ctl->work_count, -1);
+        return left == 1;
+    }
+
+    __device__ __forceinline__ void flush_counters()
+    {
+        if (lane == 0) {
+            if (dec) atomicAdd(&ir.ctl->dec, (unsigned long long)dec);
+            if (leaves) atomicAdd(&ir.ctl->leaves, (unsigned long long)leaves);
+            if (sb) atomicAdd(&ir.ctl->sb, (unsigned long long)sb);
+            if (jb) atomicAdd(&ir.ctl->jb, (unsigned long long)jb);
+        }
+        dec = leaves = sb = jb = 0;
+        if (hist && lane <= n) atomicAdd(&ir.ctl->histo[lane], (unsigned long long)hist);
+        hist = 0;
+    }
+
+    // periodic: node budget, deadline, host interrupt, incumbent refresh
+    __device__ __forceinline__ bool check_stop()
+    {
+        if (a.node_budget) {
+            unsigned long long before = w_add64(&ir.ctl->dec, (unsigned long long)dec);
+            if (before + dec >= a.node_budget && lane == 0) st_rlx(&run->stop, 1);
+            dec = 0;
+        }
+        if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
+        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(&run->stop, 1);
+        if (a.host_flag && lane == 0) {
+            int hf = __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            if (hf) st_rlx(&run->stop, 1);
+        }
+        unsigned long long b = ld_rlx(ir.best);
+        int bc = uni((int)(unsigned)(b >> 32));
+        if (bc < local_best) local_best = bc;
+        return uni(ld_rlx(&run->stop)) != 0;
+    }
+};
+
+}  // namespace pbb

@@ -14,9 +14,12 @@
 #include <map>
 #include <sstream>
 #include <string>
+#include <thread>
+#include <chrono>
 #include <vector>
 
 #include "../../include/pbb_engine.h"
+#include "pbb_internal.h"
 #include "pbb_kernels.h"
 
 using namespace pbb;
@@ -24,7 +27,7 @@ using namespace pbb;
 // ------------------------------------------------------------------ errors
 static thread_local std::string g_err;
 
-static int fail(int code, const char *fmt, ...)
+int pbb::fail(int code, const char *fmt, ...)
 {
     char buf[512];
     va_list ap;
@@ -34,14 +37,6 @@ static int fail(int code, const char *fmt, ...)
     g_err = buf;
     return code;
 }
-
-#define HIPCHK(x)                                                                      \
-    do {                                                                               \
-        hipError_t e_ = (x);                                                           \
-        if (e_ != hipSuccess)                                                          \
-            return fail(PBB_E_HIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, \
-                        __LINE__);                                                     \
-    } while (0)
 
 extern "C" const char *pbb_last_error(void) { return g_err.c_str(); }
 extern "C" const char *pbb_version(void) { return "pbbperm_amd 0.1 (gfx950)"; }
@@ -67,7 +62,7 @@ extern "C" int pbb_config_default(pbb_config *c)
     return PBB_OK;
 }
 
-static int validate(const pbb_config *c)
+int pbb::validate(const pbb_config *c)
 {
     if (c->bound_mode < 0 || c->bound_mode > 2)
         return fail(PBB_E_ARG, "boundingMode %d not in {0,1,2}", c->bound_mode);
@@ -442,7 +437,7 @@ static void free_tables(DevTables &D)
     D = DevTables{};
 }
 
-static int check_shape(const int32_t *ptm, int n, int m)
+int pbb::check_shape(const int32_t *ptm, int n, int m)
 {
     if (!ptm) return fail(PBB_E_ARG, "null ptm");
     if (n < 3 || n > PBB_MAX_JOBS) return fail(PBB_E_ARG, "n=%d outside [3, %d]", n, PBB_MAX_JOBS);
@@ -453,7 +448,7 @@ static int check_shape(const int32_t *ptm, int n, int m)
     return PBB_OK;
 }
 
-static int check_device()
+int pbb::check_device()
 {
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
@@ -502,6 +497,7 @@ static int bounds_call(pbb_bounds *b, int which, int count, const int *perms, co
     if (count == 0) return PBB_OK;
     KArgs a{};
     a.I = b->D.I;
+    a.rows = b->D.I.n;
     HIPCHK(launch_bounds(which, a, count, perms, l1, l2, arg, o1, o2, (hipStream_t)stream));
     return PBB_OK;
 }
@@ -563,6 +559,7 @@ static KArgs make_args(pbb_engine *e)
     KArgs a{};
     a.I = e->D.I;
     a.E = e->E;
+    a.rows = e->n;
     a.mat = e->mat;
     a.pos = e->pos;
     a.end = e->end;
@@ -941,4 +938,105 @@ extern "C" int pbb_query_explorers(int n, int m, const pbb_config *cfg, int *exp
     if (rc) return rc;
     *explorers = e->E;
     return pbb_destroy(e);
+}
+
+// ------------------------------------------------- plain stream replay
+// The one-instance engine re-pointed at a new instance of the same shape:
+// tables re-uploaded into the existing buffers, incumbent and root reset.
+static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub)
+{
+    build_tables(ptm, e->n, e->m, e->T);
+    const HostTables &T = e->T;
+    HIPCHK(hipMemcpy(e->D.ptm, T.ptm.data(), T.ptm.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->D.ptmT, T.ptmT.data(), T.ptmT.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->D.min_arr, T.min_arr.data(), T.m * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(e->D.min_dep, T.min_dep.data(), T.m * 4, hipMemcpyHostToDevice));
+    if (T.npairs) {
+        HIPCHK(hipMemcpy(e->D.pair_ab, T.pair_ab.data(), T.npairs * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(e->D.jtab, T.jtab.data(), T.jtab.size() * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    e->inc_cost = ub;
+    e->initial_ub = ub;
+    for (int i = 0; i < e->n; i++) e->inc_perm[i] = e->root_perm[i] = i;
+    e->root_bounded = false;
+    e->initialized = false;
+    *e->host_flag = 0;
+    unsigned long long packed = ((unsigned long long)(unsigned)ub << 32) | 0xffffffffull;
+    HIPCHK(hipMemcpy(e->best, &packed, 8, hipMemcpyHostToDevice));
+    return PBB_OK;
+}
+
+extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, const int *n,
+                                const int *m, const int *ub, const double *arrival_s, double time_budget_s,
+                                pbb_result *out, double *latency_ms, double *wall_s)
+{
+    if (!cfg || count < 0 || (count > 0 && (!ptms || !n || !m || !ub || !arrival_s || !out || !latency_ms)))
+        return fail(PBB_E_ARG, "bad replay arguments");
+    std::vector<size_t> off(count + 1, 0);
+    for (int i = 0; i < count; i++) {
+        int rc = check_shape(ptms + off[i], n[i], m[i]);
+        if (rc) return rc;
+        off[i + 1] = off[i] + (size_t)n[i] * m[i];
+    }
+    // one engine per shape, created before the clock starts
+    std::map<std::pair<int, int>, pbb_engine *> eng;
+    auto cleanup = [&]() {
+        for (auto &kv : eng) pbb_destroy(kv.second);
+    };
+    for (int i = 0; i < count; i++) {
+        auto key = std::make_pair(n[i], m[i]);
+        if (eng.count(key)) continue;
+        pbb_engine *e = nullptr;
+        int rc = pbb_create(ptms + off[i], n[i], m[i], cfg, ub[i], nullptr, &e);
+        if (rc) {
+            cleanup();
+            return rc;
+        }
+        eng[key] = e;
+    }
+    const double budget = time_budget_s > 0 ? time_budget_s : 600.0;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+    double t_last = 0.0;
+    int rc = PBB_OK;
+    for (int i = 0; i < count; i++) {
+        memset(&out[i], 0, sizeof out[i]);
+        out[i].tag = i;
+        out[i].status = PBB_RES_ABORTED;
+        out[i].best = ub[i];
+        latency_ms[i] = -1.0;
+    }
+    for (int i = 0; i < count && rc == PBB_OK; i++) {
+        while (now_s() < arrival_s[i]) std::this_thread::yield();
+        const double left = budget - now_s();
+        if (left <= 0) break;
+        pbb_engine *e = eng[std::make_pair(n[i], m[i])];
+        if ((rc = engine_retarget(e, ptms + off[i], ub[i]))) break;
+        const double t_admit = now_s();
+        int all_done = 0;
+        if ((rc = pbb_init_full(e))) break;
+        if ((rc = pbb_run(e, 0, left, &all_done))) break;
+        int cost = 0;
+        std::vector<int> perm(e->n);
+        if ((rc = pbb_get_incumbent(e, &cost, perm.data()))) break;
+        pbb_stats st;
+        if ((rc = pbb_get_stats(e, &st))) break;
+        const double tc = now_s();
+        pbb_result &o = out[i];
+        o.status = all_done ? PBB_RES_SOLVED : PBB_RES_ABORTED;
+        o.best = cost;
+        o.improved = cost < ub[i] ? 1 : 0;
+        o.decomposed = st.decomposed;
+        o.leaves = st.leaves;
+        o.simple_bounds = st.simple_bounds;
+        o.johnson_bounds = st.johnson_bounds;
+        o.admit_ms = t_admit * 1e3;
+        o.done_ms = tc * 1e3;
+        for (int j = 0; j < PBB_MAX_JOBS; j++) o.perm[j] = (o.improved && j < e->n) ? perm[j] : -1;
+        latency_ms[i] = (tc - arrival_s[i]) * 1e3;
+        t_last = tc;
+    }
+    cleanup();
+    if (wall_s) *wall_s = count ? t_last - arrival_s[0] : 0.0;
+    return rc;
 }

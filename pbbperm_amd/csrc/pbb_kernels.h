@@ -31,9 +31,27 @@ struct Ctl {
     unsigned long long histo[65];
 };
 
+// The instance one explorer works on: its control block, idle ring,
+// incumbent and bounded root row (one per engine, one per pool slot).
+struct InstRef {
+    Ctl *ctl;
+    unsigned *ring_seq;
+    int *ring_id;
+    int ring_cap;
+    int ub0;                    // initial UB (branching thresholds when not singleNode)
+    unsigned long long *best;   // cost << 32 | explorer
+    const int *root_row;
+    const int *root_dir;
+    int m;                      // machines the pair table refers to
+    int npairs;
+    int pairs_mode;
+    const int *pair_ab;
+};
+
 struct KArgs {
     DevInst I;
     int E;              // explorers (wavefronts)
+    int rows;           // rows per explorer in mat (the largest n served)
     // explorer state in HBM (row stride 64 bytes)
     int8_t *mat;        // [E][n][64]
     int8_t *pos;        // [E][64]
@@ -90,6 +108,103 @@ inline size_t root_lds_bytes(const DevInst &I)
     return lds_table_bytes(I.n, I.mp, I.npairs, false) + lds_wave_bytes(I.n, I.m);
 }
 
+// ------------------------------------------------------------------ pool
+// Many instances in one resident launch.  Workgroups of W explorers take one
+// instance slot at a time: the first group to pop a slot from the request
+// queue admits it (tables, idle ring, root), later groups join running slots
+// whose explorers are all busy, up to a per-instance explorer cap; work moves
+// between the members of a slot through its idle ring exactly as in the
+// one-instance engine.  The last member to leave a solved slot writes its
+// result record.  Instances with fewer machines than the pool's M are padded
+// with zero-time machines, which changes none of the bounds or makespans.
+enum { POOL_RUNNING = 1 << 30, POOL_DONE = 1 << 29, POOL_MEMBERS = (1 << 29) - 1 };
+enum { RES_PENDING = 0, RES_SOLVED = 1, RES_ABORTED = 2, RES_INVALID = 3 };
+enum { ROLE_EXIT = 0, ROLE_OWNER = 1, ROLE_HELPER = 2 };
+
+#define PBB_POOL_MAXM 20
+#define PBB_POOL_ROW 64
+
+struct PoolReq {                  // one request (host-visible when serving)
+    int n, m, ub, tag;
+    int ptm[PBB_POOL_MAXM * PBB_POOL_ROW];   // [m][n] machine-major, row stride n
+};
+
+struct PoolRes {                  // written once, when the instance retires
+    int tag;                      // the request's tag, stored last (completion flag)
+    int status;                   // RES_*
+    int best;                     // optimum (or the UB when no better schedule exists)
+    int improved;                 // 1 when perm holds a schedule of cost best
+    unsigned long long dec, leaves, sb, jb;
+    unsigned long long t_admit, t_done;      // device wall clock
+    int perm[PBB_POOL_ROW];
+};
+
+struct PoolQueue {                // request queue: slot ids published by tail
+    int tail;
+    int closed;
+    int pad[62];
+    int ids[1];                   // [qcap]
+};
+
+struct PoolSlot {                 // device header of one instance slot
+    Ctl c;
+    unsigned long long best;      // cost << 32 | explorer
+    unsigned long long t_admit;
+    int occ;                      // POOL_RUNNING | POOL_DONE | member explorers
+    int n, m, ub, tag, npairs, root_dir, best_cost, improved, status, pad[2];
+    int root_row[PBB_POOL_ROW];
+    int perm[PBB_POOL_ROW];
+};
+
+// byte offsets inside one slot's arena (all 256-byte aligned)
+struct PoolLayout {
+    size_t hdr, ptm, ptmT, minA, minD, pair, key, jtab, ring_seq, ring_id, stride;
+};
+__device__ __host__ inline size_t pool_al(size_t x) { return (x + 255) & ~(size_t)255; }
+__device__ __host__ inline PoolLayout pool_layout(int nmax, int M, int pmax, int ring_cap)
+{
+    const int mp = (M + 3) & ~3;
+    PoolLayout L;
+    size_t o = 0;
+    L.hdr = o;      o += pool_al(sizeof(PoolSlot));
+    L.ptm = o;      o += pool_al((size_t)M * nmax * 4);
+    L.ptmT = o;     o += pool_al((size_t)nmax * mp * 4);
+    L.minA = o;     o += pool_al((size_t)mp * 4);
+    L.minD = o;     o += pool_al((size_t)mp * 4);
+    L.pair = o;     o += pool_al((size_t)(pmax > 0 ? pmax : 1) * 4);
+    L.key = o;      o += pool_al((size_t)(pmax > 0 ? pmax : 1) * nmax * 4);
+    L.jtab = o;     o += pool_al((size_t)(pmax > 0 ? pmax : 1) * nmax * 16);
+    L.ring_seq = o; o += pool_al((size_t)ring_cap * 4);
+    L.ring_id = o;  o += pool_al((size_t)ring_cap * 4);
+    L.stride = o;
+    return L;
+}
+
+struct PoolArgs {
+    char *arena;                  // nslots x layout.stride
+    PoolLayout lay;
+    int nslots;
+    int W;                        // explorers (wavefronts) per workgroup
+    int epi;                      // explorer cap per instance
+    int nmax;                     // largest n served
+    int pmax;                     // pair-table capacity
+    int ring_cap;                 // idle-ring cells per slot (power of two >= epi)
+    int jt_lds;                   // Johnson tables staged in LDS
+    int pairs_mode;               // JohnsonMode: pair list built at admission
+    int qcap;                     // queue capacity (power of two)
+    int sys;                      // req/res/q live in host-mapped memory
+    const PoolReq *req;           // [nslots]
+    PoolRes *res;                 // [nslots]
+    PoolQueue *q;
+    unsigned *q_head;             // consumer ticket (device memory)
+    Ctl *run;                     // stop / error of the launch
+};
+
+__device__ __host__ inline size_t pool_lds_bytes(int nmax, int M, int pmax, bool jt_lds, int W)
+{
+    return lds_table_bytes(nmax, (M + 3) & ~3, pmax, jt_lds) + (size_t)W * lds_wave_bytes(nmax, M);
+}
+
 // per-M kernels (pbb_kernels.hip, one translation unit per M)
 template <int M> hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s);
 template <int M> hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu);
@@ -98,6 +213,11 @@ template <int M>
 hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perms, const int *l1,
                            const int *l2, int arg, int *o1, int *o2, hipStream_t s);
 
+// pool kernels (pbb_pool.hip, one translation unit per M)
+template <int M> hipError_t launch_pool_m(const KArgs &a, const PoolArgs &p, int groups, int init_count,
+                                          hipStream_t s);
+template <int M> hipError_t pool_occupancy_m(const KArgs &a, const PoolArgs &p, int *groups_per_cu);
+
 // runtime-M dispatch (pbb_dispatch.cpp)
 hipError_t launch_explore(const KArgs &a, int blocks, hipStream_t s);
 hipError_t explore_occupancy(const KArgs &a, int *blocks_per_cu);
@@ -105,5 +225,8 @@ hipError_t launch_root(const KArgs &a, const int *root_perm, hipStream_t s);
 // which: 0 lb1 children, 1 lb1 node, 2 lb2 node, 3 makespan
 hipError_t launch_bounds(int which, const KArgs &a, int count, const int *perms, const int *l1,
                          const int *l2, int arg, int *o1, int *o2, hipStream_t s);
+// init_count > 0: batch launch, the queue holds slots 0..init_count-1
+hipError_t launch_pool(int M, const KArgs &a, const PoolArgs &p, int groups, int init_count, hipStream_t s);
+hipError_t pool_occupancy(int M, const KArgs &a, const PoolArgs &p, int *groups_per_cu);
 
 }  // namespace pbb

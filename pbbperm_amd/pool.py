@@ -1,0 +1,218 @@
+"""Multi-instance pool: batched solves, serving loop, stream replay, autotuner.
+
+The reference solves one instance per process (its strong-bound tables are
+process-global, bounds/src/bound_fsp_strong.cpp:5-10).  The pool keeps many
+instances resident in one launch of the gfx950 pool kernel, each solved with
+the semantics of :class:`pbbperm_amd.GpuBB` (``initFullInterval`` + ``next``
+until all explorers are idle, identity root order).  Every call goes through
+the C ABI of ``libpbb_engine.so``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import Config, PoolParams, Result, check, load
+from .engine import default_config, synth, synth_shape
+
+STATUS = {0: "pending", 1: "solved", 2: "aborted", 3: "invalid"}
+
+
+@dataclass
+class PoolResult:
+    tag: int
+    status: str
+    best: int
+    improved: bool
+    decomposed: int
+    leaves: int
+    simple_bounds: int
+    johnson_bounds: int
+    admit_ms: float
+    done_ms: float
+    perm: List[int] = field(default_factory=list)
+
+
+def _result(r: Result, n: int) -> PoolResult:
+    return PoolResult(tag=r.tag, status=STATUS.get(r.status, str(r.status)), best=r.best,
+                      improved=bool(r.improved), decomposed=r.decomposed, leaves=r.leaves,
+                      simple_bounds=r.simple_bounds, johnson_bounds=r.johnson_bounds,
+                      admit_ms=r.admit_ms, done_ms=r.done_ms,
+                      perm=[r.perm[i] for i in range(n)] if r.improved else [])
+
+
+def _pack(ptms: Sequence[np.ndarray]):
+    ms = np.array([p.shape[0] for p in ptms], dtype=np.int32)
+    ns = np.array([p.shape[1] for p in ptms], dtype=np.int32)
+    flat = np.ascontiguousarray(np.concatenate([np.ascontiguousarray(p, dtype=np.int32).ravel()
+                                                for p in ptms]) if len(ptms) else np.zeros(1, np.int32))
+    return flat, ns, ms
+
+
+def _ip(a):
+    return a.ctypes.data_as(_lib.IP)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def default_params(**overrides) -> PoolParams:
+    prm = PoolParams()
+    check(load().pbb_pool_params_default(C.byref(prm)))
+    for k, v in overrides.items():
+        if not hasattr(prm, k):
+            raise KeyError(k)
+        setattr(prm, k, float(v) if k == "hbm_budget_gb" else int(v))
+    return prm
+
+
+class Pool:
+    """A resident pool of instance slots on one GPU."""
+
+    def __init__(self, config: Optional[Config] = None, params: Optional[PoolParams] = None, **kw):
+        self._lib = load()
+        self.config = config if config is not None else default_config()
+        self.params = params if params is not None else default_params(**kw)
+        h = C.c_void_p()
+        check(self._lib.pbb_pool_create(C.byref(self.config), C.byref(self.params), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.pbb_pool_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> Dict[str, int]:
+        e, g, l = C.c_int(), C.c_int(), C.c_int()
+        check(self._lib.pbb_pool_info(self._h, C.byref(e), C.byref(g), C.byref(l)))
+        return {"explorers": e.value, "groups": g.value, "lds_bytes": l.value}
+
+    def solve(self, ptms: Sequence[np.ndarray], ubs: Sequence[int], tags: Optional[Sequence[int]] = None,
+              time_budget: float = 0.0) -> Tuple[List[PoolResult], float]:
+        """Solve len(ptms) <= slots instances in one graph launch."""
+        flat, ns, ms = _pack(ptms)
+        ub = np.ascontiguousarray(ubs, dtype=np.int32)
+        tg = None if tags is None else np.ascontiguousarray(tags, dtype=np.int32)
+        out = (Result * max(1, len(ptms)))()
+        kms = C.c_double()
+        check(self._lib.pbb_pool_solve(self._h, len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms),
+                                       _ip(ub), None if tg is None else _ip(tg), float(time_budget), out,
+                                       C.byref(kms)))
+        return [_result(out[i], int(ns[i])) for i in range(len(ptms))], kms.value
+
+    # ------------------------------------------------------------ serving
+    def serve_start(self, time_budget: float):
+        check(self._lib.pbb_serve_start(self._h, float(time_budget)))
+
+    def submit(self, ptm: np.ndarray, ub: int, tag: int) -> bool:
+        """False when the request cannot be admitted now (no slot / HBM budget)."""
+        p = np.ascontiguousarray(ptm, dtype=np.int32)
+        rc = self._lib.pbb_serve_submit(self._h, p.ctypes.data_as(_lib.I32P), p.shape[1], p.shape[0],
+                                        int(ub), int(tag))
+        if rc == _lib.PBB_E_BUSY:
+            return False
+        check(rc)
+        return True
+
+    def poll(self, max_results: int = 1024) -> List[Tuple[int, Result]]:
+        out = (Result * max_results)()
+        k = C.c_int()
+        check(self._lib.pbb_serve_poll(self._h, out, max_results, C.byref(k)))
+        res = []
+        for i in range(k.value):
+            r = Result()
+            C.memmove(C.byref(r), C.byref(out[i]), C.sizeof(Result))
+            res.append(r)
+        return res
+
+    def serve_stop(self):
+        check(self._lib.pbb_serve_stop(self._h))
+
+    def footprint(self, n: int) -> int:
+        b = C.c_uint64()
+        check(self._lib.pbb_serve_footprint(self._h, int(n), C.byref(b)))
+        return b.value
+
+    def replay(self, ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Sequence[float],
+               time_budget: float = 0.0):
+        """Serve an arrival stream; returns (results, latency_ms, wall_s)."""
+        flat, ns, ms = _pack(ptms)
+        ub = np.ascontiguousarray(ubs, dtype=np.int32)
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        out = (Result * max(1, len(ptms)))()
+        lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
+        wall = C.c_double()
+        check(self._lib.pbb_serve_replay(self._h, len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms),
+                                         _ip(ub), _dp(arr), float(time_budget), out, _dp(lat), C.byref(wall)))
+        return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
+
+
+def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Sequence[float],
+                 config: Optional[Config] = None, time_budget: float = 0.0):
+    """The same stream on the one-instance engine, one launch per instance, FCFS."""
+    cfg = config if config is not None else default_config()
+    flat, ns, ms = _pack(ptms)
+    ub = np.ascontiguousarray(ubs, dtype=np.int32)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    out = (Result * max(1, len(ptms)))()
+    lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
+    wall = C.c_double()
+    check(load().pbb_plain_replay(C.byref(cfg), len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms),
+                                  _ip(ub), _dp(arr), float(time_budget), out, _dp(lat), C.byref(wall)))
+    return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
+
+
+def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Config] = None,
+             base: Optional[PoolParams] = None, max_trials: int = 64):
+    """Pick waves per workgroup, explorers per instance and the steal margin
+    that solve the given instances fastest.  Returns (params, trials) with
+    trials rows (W, explorers_per_instance, margin, ms)."""
+    cfg = config if config is not None else default_config()
+    b = base if base is not None else default_params()
+    flat, ns, ms = _pack(ptms)
+    ub = np.ascontiguousarray(ubs, dtype=np.int32)
+    best = PoolParams()
+    trials = np.zeros(4 * max_trials, dtype=np.float64)
+    nt = C.c_int()
+    check(load().pbb_pool_autotune(C.byref(cfg), C.byref(b), len(ptms), flat.ctypes.data_as(_lib.I32P),
+                                   _ip(ns), _ip(ms), _ip(ub), C.byref(best), _dp(trials), max_trials,
+                                   C.byref(nt)))
+    k = min(nt.value, max_trials)
+    return best, trials[:4 * k].reshape(k, 4)
+
+
+# ----------------------------------------------------------- workloads
+
+def instance_stream(seeds: Sequence[int], nlo: int, nhi: int, mchoices: Sequence[int]) -> List[np.ndarray]:
+    """Seeded instances of random shape: (n, m) from the shape stream of each
+    seed, times from its PTM stream (the project's SplitMix64 generator)."""
+    out = []
+    for s in seeds:
+        n, m = synth_shape(int(s), nlo, nhi, mchoices)
+        out.append(synth(int(s), n, m))
+    return out
+
+
+def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival times (s) of a Poisson process of the given rate (1/s)."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=count)
+    t = np.cumsum(gaps)
+    return t - t[0]
