@@ -1,13 +1,29 @@
-"""Throughput benchmark: decomposed nodes/s of the IVM branch-and-bound engine
-on one MI355X, on the 50 jobs x 20 machines instance (times uniform in
-[1, 99], seed 2), LB1 bounding, MaxSum branching, incumbent from NEH.
+"""Benchmark of the IVM branch-and-bound engine on one MI355X.  Prints one
+JSON line.
 
-A step is one resident-engine launch (pbb_run) exploring a fixed budget of
-decomposed nodes; the tree is far too large to finish, so the engine keeps
-its IVMs, intervals and incumbent in HBM across steps and each step
-continues the exploration.  Prints one JSON line.
+Headline (the line's top-level fields): decomposed nodes/s on the 50 jobs x
+20 machines instance (times uniform in [1, 99], seed 2), LB1 bounding,
+MaxSum branching, incumbent from NEH.  A step is one resident-engine launch
+(pbb_run) exploring a fixed budget of decomposed nodes; the tree is far too
+large to finish, so the engine keeps its IVMs, intervals and incumbent in
+HBM across steps and each step continues the exploration.
+
+Two more legs, reported as objects of the same line:
+  "batched"  the 512-instance batch (seeds 100-611, n ~ U[10,22],
+             m ~ U{5,10}, LB1 + Johnson LB2 over all machine pairs, MinMin
+             branching, NEH incumbents) solved to optimality in one
+             graph-captured launch of the pool with autotuned parameters,
+             against the plain one-instance-per-launch engine on the same
+             instances, with the reference's own code on the host cores
+             beside it;
+  "serving"  a Poisson stream of requests (n ~ U[12,30], m ~ U{5,10,20},
+             20-machine requests folded to n <= 16) through the resident
+             serving loop and through the plain path, same stream, same
+             arrival times: solves/s and p50/p99 latency.
+Every optimum of both legs is compared between the pool and the plain path.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--node-budget B]
+                  [--legs nodes,batched,serving] [--serve-requests N]
 """
 from __future__ import annotations
 
@@ -56,6 +72,142 @@ def cpu_baseline(ptm, ub, seconds: float):
                       f"({r.decomposed} decomposed nodes)"}
 
 
+def note(msg: str):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def _pct(a, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(a, dtype=float), q)) if len(a) else None
+
+
+def cpu_solves(ptms, ubs, kw, seconds: float):
+    """The reference's own code (oracle/_ref, one process per instance, as
+    its process-global bound tables require) over the host cores on a
+    bounded sample: instances in order until the time is up; solves/s =
+    instances finished / wall."""
+    import concurrent.futures as cf
+    import oracle as O
+    O.lib()
+    kind = "reference" if O.have_ref() else "port"
+    cores = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.time()
+    done = 0
+
+    def run(i):
+        left = seconds - (time.time() - t0)
+        if left <= 0.05:
+            return False
+        if kind == "reference":
+            r = O.ref_solve(ptms[i], ubs[i], time_limit=left, **kw)
+        else:
+            r = O.solve(ptms[i], ubs[i], **kw)
+        return bool(r.finished) and time.time() - t0 <= seconds * 1.05
+
+    with cf.ThreadPoolExecutor(cores) as ex:
+        for ok in ex.map(run, range(len(ptms))):
+            done += int(ok)
+    wall = time.time() - t0
+    return {"value": done / wall, "unit": "solves/s", "cores": cores, "kind": kind,
+            "sample": f"{done} of the same instances in order, one process each, {wall:.1f} s"}
+
+
+def leg_batched(P, PP, args):
+    import numpy as np
+    cfg = P.default_config(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_johnson=0,
+                           nb_ivms=4096)
+    ptms = PP.batch_stream(512, 100)
+    ubs = [P.neh(p)[0] for p in ptms]
+    # autotuned on a disjoint sample of the same distribution
+    tune = PP.batch_stream(128, 1100)
+    tune_ubs = [P.neh(p)[0] for p in tune]
+    base = PP.default_params(max_jobs=22, machines=10, slots=512)
+    t0 = time.perf_counter()
+    note("batched: autotuning on 128 instances")
+    best, trials = PP.autotune(tune, tune_ubs, config=cfg, base=base, progress=lambda r: note(f"  trial {r}"))
+    tune_s = time.perf_counter() - t0
+    best.slots = 512
+    note(f"batched: 512 instances with W={best.waves_per_group} epi={best.explorers_per_instance} "
+         f"margin={best.steal_margin}")
+    with PP.Pool(cfg, params=best) as pool:
+        res, _ = pool.solve(ptms, ubs, time_budget=120.0)
+        walls, kms = [], []
+        for _ in range(3):
+            t = time.perf_counter()
+            res, km = pool.solve(ptms, ubs, time_budget=120.0)
+            walls.append(time.perf_counter() - t)
+            kms.append(km)
+        info = pool.info()
+    note(f"batched: pool {min(walls) * 1e3:.1f} ms; plain path")
+    plain, _, pwall = PP.plain_replay(ptms, ubs, np.zeros(len(ptms)), config=cfg, time_budget=300.0)
+    note(f"batched: plain {pwall * 1e3:.1f} ms")
+    ok = all(r.status == "solved" for r in res) and all(r.status == "solved" for r in plain)
+    same = all(a.best == b.best for a, b in zip(res, plain))
+    ref_match = None
+    fx = os.path.join(ROOT, "tests", "golden", "batch_c4.json")
+    if os.path.exists(fx):
+        with open(fx) as f:
+            opts = [d["opt"] for d in json.load(f)["instances"]]
+        ref_match = [r.best for r in res] == opts
+    wall = min(walls)
+    nodes = sum(r.decomposed for r in res)
+    out = {
+        "workload": "512 instances, seeds 100-611, n~U[10,22], m~U{5,10}, LB1 + Johnson LB2 (all pairs), "
+                    "MinMin branching, NEH incumbents, solved to optimality",
+        "solves_per_s": len(ptms) / wall, "wall_ms": wall * 1e3, "graph_ms": min(kms),
+        "decomposed": nodes, "nodes_per_s": nodes / wall,
+        "plain_solves_per_s": len(ptms) / pwall, "plain_wall_ms": pwall * 1e3,
+        "speedup_vs_plain": pwall / wall,
+        "all_solved": ok, "optima_equal_plain": same, "optima_equal_reference": ref_match,
+        "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
+                  "steal_margin": best.steal_margin, "trials": int(len(trials)), "tune_s": tune_s},
+        "explorers": info["explorers"],
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_solves(ptms, ubs, dict(bound_mode=2, branching=3, refine=1, johnson_pairs=0),
+                                         args.cpu_seconds)
+    return out
+
+
+def leg_serving(P, PP, args):
+    import numpy as np
+    N = args.serve_requests
+    cfg = P.default_config(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_johnson=0,
+                           nb_ivms=4096)
+    ptms = PP.serving_stream(N, 10000)
+    ubs = [P.neh(p)[0] for p in ptms]
+    tune = PP.serving_stream(128, 30000)
+    tune_ubs = [P.neh(p)[0] for p in tune]
+    base = PP.default_params(max_jobs=30, machines=20, slots=256)
+    note("serving: autotuning on 128 requests")
+    best, trials = PP.autotune(tune, tune_ubs, config=cfg, base=base, progress=lambda r: note(f"  trial {r}"))
+    best.slots = 256
+    # arrival rate: a fraction of the plain path's closed-loop rate
+    k = min(N, 300)
+    _, _, w0 = PP.plain_replay(ptms[:k], ubs[:k], np.zeros(k), config=cfg, time_budget=300.0)
+    rate = args.serve_load * k / w0
+    arrivals = PP.poisson_arrivals(N, rate, seed=7)
+    note(f"serving: {N} requests at {rate:.1f}/s through the pool")
+    with PP.Pool(cfg, params=best) as pool:
+        res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=900.0)
+    note(f"serving: pool done in {wall:.1f} s; plain path")
+    plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=cfg, time_budget=900.0)
+    note(f"serving: plain done in {pwall:.1f} s")
+    ok = all(r.status == "solved" for r in res) and all(r.status == "solved" for r in plain)
+    same = all(a.best == b.best for a, b in zip(res, plain))
+    return {
+        "workload": f"{N} Poisson requests, seeds 10000+, n~U[12,30], m~U{{5,10,20}} (m=20 folded to n<=16), "
+                    "LB1 + Johnson LB2, MinMin, NEH incumbents",
+        "rate_per_s": rate, "load_vs_plain": args.serve_load,
+        "solves_per_s": N / wall, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "max_ms": float(np.max(lat)),
+        "plain": {"solves_per_s": N / pwall, "p50_ms": _pct(plat, 50), "p99_ms": _pct(plat, 99),
+                  "max_ms": float(np.max(plat))},
+        "all_solved": ok, "optima_equal_plain": same,
+        "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
+                  "steal_margin": best.steal_margin},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +217,10 @@ def main():
     ap.add_argument("--explorers", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="nodes,batched,serving")
+    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-load", type=float, default=0.8,
+                    help="arrival rate as a fraction of the plain path's closed-loop solves/s")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run a plain device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -140,6 +296,15 @@ def main():
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ptm, ub, args.cpu_seconds)
+    legs = set(args.legs.split(","))
+    note(f"nodes leg: {value / 1e6:.1f} M nodes/s")
+    if legs & {"batched", "serving"}:
+        import pbbperm_amd.pool as PP
+        if "batched" in legs:
+            line["batched"] = leg_batched(P, PP, args)
+            print(json.dumps({"leg": "batched", **line["batched"]}), file=sys.stderr, flush=True)
+        if "serving" in legs:
+            line["serving"] = leg_serving(P, PP, args)
     print(json.dumps(line), flush=True)
 
 

@@ -230,6 +230,10 @@ int pbb_pool_destroy(pbb_pool *p);
 /* Explorers and workgroups the pool launches. */
 int pbb_pool_info(pbb_pool *p, int *explorers, int *groups, int *lds_bytes);
 
+/* Change the per-instance explorer cap and the steal margin of an idle pool
+ * (the cap stays within the idle-ring capacity chosen at creation). */
+int pbb_pool_retune(pbb_pool *p, int explorers_per_instance, int steal_margin);
+
 /* Batch: count <= slots instances, instance i's m[i] x n[i] machine-major
  * times at ptms[off_i], off_i = sum of n[k]*m[k] for k < i.  Upload, reset,
  * the resident launch and the result download are one graph launch.
@@ -263,7 +267,8 @@ int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, cons
                      pbb_result *out, double *latency_ms, double *wall_s);
 
 /* Autotuner: time pbb_pool_solve on the given instances for every
- * (waves_per_group, explorers_per_instance, steal_margin) candidate and
+ * (waves_per_group in {1,2,4}, explorers_per_instance in {32,256,2048},
+ * steal_margin in {2,4}) candidate and
  * write the fastest into *best (other fields taken from *base).  trials
  * (may be NULL) receives up to max_trials rows {W, epi, margin, ms}. */
 int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *base, int count,

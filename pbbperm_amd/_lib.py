@@ -111,6 +111,7 @@ SIGNATURES = {
     "pbb_pool_create": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.POINTER(VP)]),
     "pbb_pool_destroy": (C.c_int, [VP]),
     "pbb_pool_info": (C.c_int, [VP, IP, IP, IP]),
+    "pbb_pool_retune": (C.c_int, [VP, C.c_int, C.c_int]),
     "pbb_pool_solve": (C.c_int, [VP, C.c_int, I32P, IP, IP, IP, IP, C.c_double, C.POINTER(Result),
                                  C.POINTER(C.c_double)]),
     "pbb_serve_start": (C.c_int, [VP, C.c_double]),

@@ -790,6 +790,10 @@ struct Explorer {
     // ivm.cpp:188-237): split at the first line l whose pos differs from end
     // (l <= L, l < n - margin); the thief gets the explorable cells after the
     // victim's kept half, rows 0..l and the victim's end; no re-bounding.
+    // A line whose cells after pos are all pruned or visited holds no work:
+    // its end is trimmed to pos (deeper ends become the maximum, so the
+    // victim's interval covers exactly the same live nodes) and the split
+    // moves down to the next line.  Node counts are unchanged by either step.
     // h, t: idle-ring head/tail loaded before the node's work (prefetch).
     __device__ __forceinline__ void try_donate(unsigned h, unsigned t)
     {
@@ -801,13 +805,26 @@ struct Explorer {
         uint64_t ne = ballot(lane < n && posv != endv);
         uint64_t limmask = lim >= 64 ? ~0ull : ((1ull << lim) - 1ull);
         uint64_t cand = ne & limmask;
-        int l = cand ? __ffsll((unsigned long long)cand) - 1 : lim;
-        int pl = rdlane(posv, l), el = rdlane(endv, l);
-        if (!(pl < el)) return;
-        int rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[l * 64 + lane] : -1);
-        uint64_t expl = ballot(lane > pl && lane <= el && rowl >= 0);
+        const int first = cand ? __ffsll((unsigned long long)cand) - 1 : lim;
+        int l = first, pl = 0, el = 0, rowl = -1;
+        uint64_t expl = 0ull;
+        for (;; l++) {
+            if (l > lim) return;
+            pl = rdlane(posv, l);
+            el = (l == first) ? rdlane(endv, l) : n - 1 - l;
+            if (pl < el) {
+                rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[l * 64 + lane] : -1);
+                expl = ballot(lane > pl && lane <= el && rowl >= 0);
+                if (expl) break;
+            }
+        }
+        // the victim's end once lines first..l-1 are trimmed
+        int endE = endv;
+        if (l > first) {
+            if (lane >= first && lane < l) endE = posv;
+            else if (lane >= l && lane < n) endE = n - 1 - lane;
+        }
         int nexp = __popcll(expl);
-        if (nexp < 1) return;
         if (!w_cas(&ir.ctl->idle_head, h, h + 1)) return;
         // claim the idle explorer registered at ticket h (bounded MPMC ring cell)
         const unsigned slot = h & (unsigned)(ir.ring_cap - 1);
@@ -844,11 +861,12 @@ struct Explorer {
         if (lane < n) gm[l * 64 + lane] = (int8_t)rowl;
         if (lane < n) {
             a.pos[(size_t)thief * 64 + lane] = (int8_t)(lane < l ? posv : (lane == l ? cut : 0));
-            a.end[(size_t)thief * 64 + lane] = (int8_t)endv;
+            a.end[(size_t)thief * 64 + lane] = (int8_t)endE;
             a.dir[(size_t)thief * 64 + lane] = (int8_t)(lane <= l ? dirv : 0);
         }
         if (lane == 0) a.line[thief] = l;
         // the victim keeps [pos, cut-1] at line l and everything below
+        endv = endE;
         if (lane == l) endv = cut - 1;
         if (lane > l && lane < n) endv = n - 1 - lane;
         w_add(&ir.ctl->work_count, 1);

@@ -302,6 +302,21 @@ extern "C" int pbb_pool_info(pbb_pool *p, int *explorers, int *groups, int *lds_
     return PBB_OK;
 }
 
+extern "C" int pbb_pool_retune(pbb_pool *p, int explorers_per_instance, int steal_margin)
+{
+    if (!p) return fail(PBB_E_ARG, "null pool");
+    if (p->serving) return fail(PBB_E_STATE, "pool is serving");
+    if (explorers_per_instance < p->W || steal_margin < 0)
+        return fail(PBB_E_ARG, "explorers_per_instance %d / steal_margin %d", explorers_per_instance, steal_margin);
+    int epi = std::min(explorers_per_instance - explorers_per_instance % p->W, p->E);
+    if (2 * epi > p->ring_cap)
+        return fail(PBB_E_CAPACITY, "explorers_per_instance %d above the pool's idle-ring capacity %d", epi,
+                    p->ring_cap / 2);
+    p->epi = epi;
+    p->margin = steal_margin ? steal_margin : p->cfg.steal_margin;
+    return PBB_OK;
+}
+
 // validate one instance of a request list and copy it into a request record
 static int fill_req(const pbb_pool *p, PoolReq *r, const int32_t *ptm, int n, int m, int ub, int tag)
 {
@@ -582,8 +597,8 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
     if (!cfg || !base || !best || count < 1 || !ptms || !n || !m || !ub)
         return fail(PBB_E_ARG, "bad autotune arguments");
     const int Ws[] = {1, 2, 4};
-    const int epis[] = {16, 64, 256, 1024, 4096};
-    const int margins[] = {2, 4, 6};
+    const int epis[] = {32, 256, 2048};
+    const int margins[] = {2, 4};
     std::vector<pbb_result> res(count);
     double best_ms = 1e300;
     int nt = 0;
@@ -599,10 +614,9 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
         for (int epi : epis) {
             if (epi < W || epi > p->E) continue;
             for (int mg : margins) {
-                p->epi = epi - epi % W;
-                p->margin = mg;
+                rc = pbb_pool_retune(p, epi, mg);
                 double ms = 0.0;
-                rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, 0.0, res.data(), &ms);  // warm
+                if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, 0.0, res.data(), &ms);  // warm
                 if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, 0.0, res.data(), &ms);
                 if (rc) {
                     pbb_pool_destroy(p);

@@ -12,8 +12,9 @@
 //                      bounds the root (ivm_bound::boundRoot,
 //                      ivm_bound.cpp:273-330) and explores the full interval
 //                      from explorer 0 of the group;
-//                    * helper: joins a running slot whose explorers are all
-//                      busy (up to the per-instance cap) and receives work
+//                    * helper: joins the running slot with the most busy
+//                      explorers net of idle ones (up to the per-instance
+//                      cap) and receives work
 //                      through the slot's idle ring, as explorers of the
 //                      one-instance engine do.
 //                  The explorer whose release empties a slot records the
@@ -217,9 +218,9 @@ static __device__ bool pool_admit(const PoolArgs &p, int s, bool lb2)
 }
 
 // -------------------------------------------------------- slot selection
-// Wave 0 of a free workgroup: a new instance from the queue first, else a
-// running slot whose explorers are all busy (the largest such team), else
-// wait.  Returns the role; *slot receives the slot id.
+// Wave 0 of a free workgroup: a new instance from the queue first, else the
+// running slot with the largest surplus of busy over idle explorers (below
+// the per-instance cap), else wait.  Returns the role; *slot receives the slot id.
 static __device__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_end, const int *host_flag, int *slot)
 {
     const int lane = lane_id();
@@ -246,9 +247,10 @@ static __device__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_e
             const bool live = (o & POOL_RUNNING) && !(o & POOL_DONE);
             if (live) wc = ld_rlx(&slot_hdr(p, s)->c.work_count);
             const int mem = o & POOL_MEMBERS;
-            const bool cand = live && mem + W <= p.epi && wc >= mem;
+            // more busy than idle members: work is there to be split
+            const bool cand = live && mem + W <= p.epi && 2 * wc > mem;
             running = running || ballot(live) != 0ull;
-            const int sc = cand ? mem : -1;
+            const int sc = cand ? 2 * wc - mem : -1;
             const int mx = wave_max(sc);
             if (mx > best_score) {
                 const int l = __ffsll((unsigned long long)ballot(sc == mx)) - 1;

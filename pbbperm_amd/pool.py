@@ -99,6 +99,9 @@ class Pool:
     def __exit__(self, *exc):
         self.close()
 
+    def retune(self, explorers_per_instance: int, steal_margin: int):
+        check(self._lib.pbb_pool_retune(self._h, int(explorers_per_instance), int(steal_margin)))
+
     def info(self) -> Dict[str, int]:
         e, g, l = C.c_int(), C.c_int(), C.c_int()
         check(self._lib.pbb_pool_info(self._h, C.byref(e), C.byref(g), C.byref(l)))
@@ -179,11 +182,52 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
     return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
 
 
+GRID_W = (1, 2, 4)
+GRID_EPI = (32, 256, 2048)
+GRID_MARGIN = (2, 4)
+
+
 def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Config] = None,
-             base: Optional[PoolParams] = None, max_trials: int = 64):
-    """Pick waves per workgroup, explorers per instance and the steal margin
-    that solve the given instances fastest.  Returns (params, trials) with
-    trials rows (W, explorers_per_instance, margin, ms)."""
+             base: Optional[PoolParams] = None, time_budget: float = 120.0, progress=None):
+    """Pick waves per workgroup (block size), explorers per instance and the
+    steal margin that solve the given instances fastest: every candidate of
+    GRID_W x GRID_EPI x GRID_MARGIN is timed on one warm graph launch.
+    Returns (params, trials) with trials rows (W, explorers_per_instance,
+    margin, ms).  progress(row) is called after each trial."""
+    cfg = config if config is not None else default_config()
+    b = base if base is not None else default_params()
+    trials = []
+    for W in GRID_W:
+        prm = PoolParams()
+        C.memmove(C.byref(prm), C.byref(b), C.sizeof(PoolParams))
+        prm.waves_per_group = W
+        prm.explorers_per_instance = max(GRID_EPI)
+        prm.slots = max(b.slots, len(ptms))
+        with Pool(cfg, params=prm) as pool:
+            E = pool.info()["explorers"]
+            for epi in GRID_EPI:
+                if epi < W or epi > E:
+                    continue
+                for mg in GRID_MARGIN:
+                    pool.retune(epi, mg)
+                    pool.solve(ptms, ubs, time_budget=time_budget)
+                    res, ms = pool.solve(ptms, ubs, time_budget=time_budget)
+                    if not all(r.status == "solved" for r in res):
+                        ms = float("inf")
+                    row = (W, epi, mg, ms)
+                    trials.append(row)
+                    if progress:
+                        progress(row)
+    W, epi, mg, _ = min(trials, key=lambda r: r[3])
+    best = PoolParams()
+    C.memmove(C.byref(best), C.byref(b), C.sizeof(PoolParams))
+    best.waves_per_group, best.explorers_per_instance, best.steal_margin = W, epi, mg
+    return best, np.array(trials, dtype=np.float64)
+
+
+def autotune_native(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Config] = None,
+                    base: Optional[PoolParams] = None, max_trials: int = 64):
+    """The same search run inside the library (pbb_pool_autotune)."""
     cfg = config if config is not None else default_config()
     b = base if base is not None else default_params()
     flat, ns, ms = _pack(ptms)
@@ -207,6 +251,29 @@ def instance_stream(seeds: Sequence[int], nlo: int, nhi: int, mchoices: Sequence
     for s in seeds:
         n, m = synth_shape(int(s), nlo, nhi, mchoices)
         out.append(synth(int(s), n, m))
+    return out
+
+
+def batch_stream(count: int = 512, seed0: int = 100) -> List[np.ndarray]:
+    """The batched workload: seeds seed0.., n ~ U[10,22], m ~ U{5,10}."""
+    return instance_stream(range(seed0, seed0 + count), 10, 22, (5, 10))
+
+
+SERVE_NMAX_M20 = 16
+
+
+def serving_stream(count: int, seed0: int = 10000) -> List[np.ndarray]:
+    """The serving workload: request i has seed seed0+i, n ~ U[12,30] and
+    m ~ U{5,10,20} from its shape stream.  Requests with 20 machines are
+    folded to n in [12,16] (n = 12 + (n - 12) mod 5): 20-machine instances of
+    this generator with n >= 17 need more than 3M nodes with LB1 + LB2
+    (35 of 39 sampled on the CPU oracle), far beyond a request's latency."""
+    out = []
+    for s in range(seed0, seed0 + count):
+        n, m = synth_shape(s, 12, 30, (5, 10, 20))
+        if m == 20:
+            n = 12 + (n - 12) % (SERVE_NMAX_M20 - 11)
+        out.append(synth(s, n, m))
     return out
 
 

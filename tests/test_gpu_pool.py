@@ -276,7 +276,10 @@ def test_autotuner_picks_a_valid_configuration(oracle_mod):
     ptms = _small_batch(O, 24, 7500)
     opts = [_opt(O, p) for p in ptms]
     base = PP.default_params(max_jobs=12, machines=5, slots=24)
-    best, trials = PP.autotune(ptms, opts, config=_cfg(P, 2, 3, 1, 0), base=base)
+    best, trials = PP.autotune(ptms, opts, config=_cfg(P, 2, 3, 1, 0), base=base, time_budget=BUDGET)
+    nbest, ntrials = PP.autotune_native(ptms, opts, config=_cfg(P, 2, 3, 1, 0), base=base)
+    assert len(ntrials) == len(trials)
+    assert [tuple(r[:3]) for r in ntrials] == [tuple(r[:3]) for r in trials]
     assert best.waves_per_group in (1, 2, 4)
     assert best.explorers_per_instance >= best.waves_per_group
     assert len(trials) >= 9
