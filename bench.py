@@ -17,9 +17,12 @@ Two more legs, reported as objects of the same line:
              instances, with the reference's own code on the host cores
              beside it;
   "serving"  a Poisson stream of requests (n ~ U[12,30], m ~ U{5,10,20},
-             20-machine requests folded to n <= 16) through the resident
-             serving loop and through the plain path, same stream, same
-             arrival times: solves/s and p50/p99 latency.
+             20-machine requests folded to n <= 16; 600 by default, the
+             10,000-request stream with --serve-requests 10000) through the
+             resident serving loop and through the plain path, same stream,
+             same arrival times: solves/s and p50/p99 latency.
+Pool parameters come from the autotuner's stored choice
+(pbbperm_amd/tuned.json, python -m pbbperm_amd.tune), or --autotune.
 Every optimum of both legs is compared between the pool and the plain path.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--node-budget B]
@@ -112,22 +115,26 @@ def cpu_solves(ptms, ubs, kw, seconds: float):
             "sample": f"{done} of the same instances in order, one process each, {wall:.1f} s"}
 
 
+def _params(PP, name, args):
+    """Pool parameters of a workload: the autotuner's stored choice
+    (pbbperm_amd/tuned.json, written by python -m pbbperm_amd.tune), or a
+    fresh autotuning run with --autotune or when none is stored."""
+    best = None if args.autotune else PP.tuned_params(name)
+    src = "stored autotune (pbbperm_amd/tuned.json)"
+    if best is None:
+        note(f"{name}: autotuning")
+        best, _ = PP.tune_workload(name, progress=lambda r: note(f"  trial {r}"))
+        src = "autotuned in this run"
+    return best, src
+
+
 def leg_batched(P, PP, args):
     import numpy as np
-    cfg = P.default_config(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_johnson=0,
-                           nb_ivms=4096)
+    cfg = PP.workload_config("batched")
     ptms = PP.batch_stream(512, 100)
     ubs = [P.neh(p)[0] for p in ptms]
-    # autotuned on a disjoint sample of the same distribution
-    tune = PP.batch_stream(128, 1100)
-    tune_ubs = [P.neh(p)[0] for p in tune]
-    base = PP.default_params(max_jobs=22, machines=10, slots=512)
-    t0 = time.perf_counter()
-    note("batched: autotuning on 128 instances")
-    best, trials = PP.autotune(tune, tune_ubs, config=cfg, base=base, progress=lambda r: note(f"  trial {r}"))
-    tune_s = time.perf_counter() - t0
-    best.slots = 512
-    note(f"batched: 512 instances with W={best.waves_per_group} epi={best.explorers_per_instance} "
+    best, src = _params(PP, "batched", args)
+    note(f"batched: 512 instances, W={best.waves_per_group} epi={best.explorers_per_instance} "
          f"margin={best.steal_margin}")
     with PP.Pool(cfg, params=best) as pool:
         res, _ = pool.solve(ptms, ubs, time_budget=120.0)
@@ -139,7 +146,9 @@ def leg_batched(P, PP, args):
             kms.append(km)
         info = pool.info()
     note(f"batched: pool {min(walls) * 1e3:.1f} ms; plain path")
-    plain, _, pwall = PP.plain_replay(ptms, ubs, np.zeros(len(ptms)), config=cfg, time_budget=300.0)
+    # the plain path gets the same per-instance explorer count
+    pcfg = PP.workload_config("batched", nb_ivms=best.explorers_per_instance)
+    plain, _, pwall = PP.plain_replay(ptms, ubs, np.zeros(len(ptms)), config=pcfg, time_budget=300.0)
     note(f"batched: plain {pwall * 1e3:.1f} ms")
     ok = all(r.status == "solved" for r in res) and all(r.status == "solved" for r in plain)
     same = all(a.best == b.best for a, b in zip(res, plain))
@@ -153,17 +162,19 @@ def leg_batched(P, PP, args):
     nodes = sum(r.decomposed for r in res)
     out = {
         "workload": "512 instances, seeds 100-611, n~U[10,22], m~U{5,10}, LB1 + Johnson LB2 (all pairs), "
-                    "MinMin branching, NEH incumbents, solved to optimality",
+                    "MinMin branching, NEH incumbents, solved to optimality in one graph launch",
         "solves_per_s": len(ptms) / wall, "wall_ms": wall * 1e3, "graph_ms": min(kms),
         "decomposed": nodes, "nodes_per_s": nodes / wall,
         "plain_solves_per_s": len(ptms) / pwall, "plain_wall_ms": pwall * 1e3,
+        "plain_explorers_per_instance": best.explorers_per_instance,
         "speedup_vs_plain": pwall / wall,
         "all_solved": ok, "optima_equal_plain": same, "optima_equal_reference": ref_match,
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
-                  "steal_margin": best.steal_margin, "trials": int(len(trials)), "tune_s": tune_s},
+                  "steal_margin": best.steal_margin, "source": src},
         "explorers": info["explorers"],
     }
     if not args.no_cpu:
+        note("batched: reference on the host cores")
         out["cpu_baseline"] = cpu_solves(ptms, ubs, dict(bound_mode=2, branching=3, refine=1, johnson_pairs=0),
                                          args.cpu_seconds)
     return out
@@ -172,39 +183,35 @@ def leg_batched(P, PP, args):
 def leg_serving(P, PP, args):
     import numpy as np
     N = args.serve_requests
-    cfg = P.default_config(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_johnson=0,
-                           nb_ivms=4096)
+    cfg = PP.workload_config("serving")
     ptms = PP.serving_stream(N, 10000)
     ubs = [P.neh(p)[0] for p in ptms]
-    tune = PP.serving_stream(128, 30000)
-    tune_ubs = [P.neh(p)[0] for p in tune]
-    base = PP.default_params(max_jobs=30, machines=20, slots=256)
-    note("serving: autotuning on 128 requests")
-    best, trials = PP.autotune(tune, tune_ubs, config=cfg, base=base, progress=lambda r: note(f"  trial {r}"))
-    best.slots = 256
+    best, src = _params(PP, "serving", args)
+    pcfg = PP.workload_config("serving", nb_ivms=best.explorers_per_instance)
     # arrival rate: a fraction of the plain path's closed-loop rate
-    k = min(N, 300)
-    _, _, w0 = PP.plain_replay(ptms[:k], ubs[:k], np.zeros(k), config=cfg, time_budget=300.0)
+    k = min(N, 200)
+    _, _, w0 = PP.plain_replay(ptms[:k], ubs[:k], np.zeros(k), config=pcfg, time_budget=300.0)
     rate = args.serve_load * k / w0
     arrivals = PP.poisson_arrivals(N, rate, seed=7)
     note(f"serving: {N} requests at {rate:.1f}/s through the pool")
     with PP.Pool(cfg, params=best) as pool:
         res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=900.0)
     note(f"serving: pool done in {wall:.1f} s; plain path")
-    plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=cfg, time_budget=900.0)
+    plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=pcfg, time_budget=900.0)
     note(f"serving: plain done in {pwall:.1f} s")
     ok = all(r.status == "solved" for r in res) and all(r.status == "solved" for r in plain)
     same = all(a.best == b.best for a, b in zip(res, plain))
     return {
         "workload": f"{N} Poisson requests, seeds 10000+, n~U[12,30], m~U{{5,10,20}} (m=20 folded to n<=16), "
-                    "LB1 + Johnson LB2, MinMin, NEH incumbents",
+                    "LB1 + Johnson LB2, MinMin, NEH incumbents, every request solved to optimality",
         "rate_per_s": rate, "load_vs_plain": args.serve_load,
         "solves_per_s": N / wall, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "max_ms": float(np.max(lat)),
+        "decomposed": int(sum(r.decomposed for r in res)),
         "plain": {"solves_per_s": N / pwall, "p50_ms": _pct(plat, 50), "p99_ms": _pct(plat, 99),
-                  "max_ms": float(np.max(plat))},
+                  "max_ms": float(np.max(plat)), "explorers_per_instance": best.explorers_per_instance},
         "all_solved": ok, "optima_equal_plain": same,
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
-                  "steal_margin": best.steal_margin},
+                  "steal_margin": best.steal_margin, "source": src},
     }
 
 
@@ -218,7 +225,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--legs", default="nodes,batched,serving")
-    ap.add_argument("--serve-requests", type=int, default=10000)
+    ap.add_argument("--serve-requests", type=int, default=600)
+    ap.add_argument("--autotune", action="store_true", help="re-run the pool autotuner instead of tuned.json")
     ap.add_argument("--serve-load", type=float, default=0.8,
                     help="arrival rate as a fraction of the plain path's closed-loop solves/s")
     ap.add_argument("--copy-probe", type=int, default=0,

@@ -180,7 +180,9 @@ __device__ __forceinline__ InstRef engine_inst(const KArgs &a)
 // never overlap.  A chain started from min_arr/min_dep equals one started
 // from zero, since min_arr[q] never exceeds a single job's head (and min_dep
 // its tail).
-template <int M, bool REF>
+// REF: 0 LB1 only (boundingMode 0); 1 LB1 + LB1' refinement (boundingMode
+// 1/2 as shipped); 2 LB1 + Johnson LB2 refinement.
+template <int M, int REF>
 struct Explorer {
     const KArgs &a;
     Lds lds;
@@ -392,7 +394,7 @@ struct Explorer {
         const bool both = a.branching > 0;
         int dir, c1, c2 = 0;
         bool refined = false;
-        if constexpr (!REF) {
+        if constexpr (REF == 0) {
             int cb, ce;
             lb1_children(k, cb, ce, nullptr);
             if (!both) ce = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
@@ -488,51 +490,50 @@ struct Explorer {
     }
 
     // Johnson LB2 of child c (uniform), direction d
-    // (bound_fsp_strong::bornes_calculer with the child's partial schedule)
-    __device__ __forceinline__ int lb2_child(int c, int d, const int (&chF)[M], const int (&chB)[M],
-                                             uint64_t rowmask, int k)
+    // (bound_fsp_strong::bornes_calculer with the child's partial schedule).
+    // The child's front (d = 0) or back (d = 1) completion times come from
+    // one lane-parallel chain scan (lane = machine, as a stack push) and go
+    // to the wave's LDS scratch, where the pair lanes read them.
+    __device__ __forceinline__ int lb2_child(int c, int d, uint64_t rowmask, int k)
     {
         const int cj = flag_abs(rdlane(rowv, c));
+        int *fs = lds.scr, *bs = lds.scr + 32;
+        if (d == 0) {
+            const int f = lane < M ? frow(nf)[lane] : 0;
+            const int cf = chain_scan(f, pmach(cj));
+            if (lane < M) {
+                fs[lane] = cf;
+                bs[lane] = brow(nb)[lane];
+            }
+        } else {
+            // back chain over machines M-1..0: lane q handles machine M-1-q
+            const int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
+            const int pr = lane < M ? lds.ptmT[cj * mp + (M - 1 - lane)] : 0;
+            const int cb = chain_scan(base, pr);
+            if (lane < M) {
+                fs[lane] = frow(nf)[lane];
+                bs[M - 1 - lane] = cb;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        int r;
         if (k <= 2) {
             // l2 - l1 <= 2: makespan of the completed permutation (bound_fsp_strong.cpp:543-546)
             const int other = flag_abs(rdlane(rowv, 1 - c));
-            int fr[M];
-#pragma unroll
-            for (int q = 0; q < M; q++) fr[q] = d == 0 ? rdlane(chF[q], c) : fval(q);
-            int po = pmach(other);
-            int cc = 0, best = 0;
-#pragma unroll
-            for (int q = 0; q < M; q++) {
-                cc = (q == 0 ? fr[0] : max(fr[q], cc)) + rdlane(po, q);
-                int bk = d == 1 ? rdlane(chB[q], c) : bval(q);
-                best = max(best, cc + bk);
-            }
-            return best;
+            const int fq = lane < M ? fs[lane] : 0;
+            const int bq = lane < M ? bs[lane] : 0;
+            const int cc = chain_scan(fq, pmach(other));
+            r = wave_max(lane < M ? cc + bq : 0);
+        } else {
+            const uint64_t unsched = rowmask & ~(1ull << cj);
+            DevInst J{};
+            J.n = n;
+            J.m = ir.m;
+            J.npairs = ir.npairs;
+            J.pair_ab = ir.pair_ab;
+            J.jtab = lds.jtab;
+            r = lb2_pairs_wave(J, fs, bs, unsched, ir.pairs_mode);
         }
-        int *fs = lds.scr, *bs = lds.scr + 32;
-        int fl = 0, bl = 0;
-#pragma unroll
-        for (int q = 0; q < M; q++) {
-            int fq = d == 0 ? rdlane(chF[q], c) : fval(q);
-            int bq = d == 1 ? rdlane(chB[q], c) : bval(q);
-            if (lane == q) {
-                fl = fq;
-                bl = bq;
-            }
-        }
-        if (lane < M) {
-            fs[lane] = fl;
-            bs[lane] = bl;
-        }
-        __builtin_amdgcn_wave_barrier();
-        uint64_t unsched = rowmask & ~(1ull << cj);
-        DevInst J{};
-        J.n = n;
-        J.m = ir.m;
-        J.npairs = ir.npairs;
-        J.pair_ab = ir.pair_ab;
-        J.jtab = lds.jtab;
-        int r = lb2_pairs_wave(J, fs, bs, unsched, ir.pairs_mode);
         __builtin_amdgcn_wave_barrier();
         return r;
     }
@@ -545,9 +546,9 @@ struct Explorer {
         bool todo = valid && !gt_best(cw);
         uint64_t tmask = ballot(todo);
         jb += (unsigned)__popcll(tmask);
-        int chF[M], chB[M];
-        child_chains(job, chF, chB);
-        if (a.refine == 0) {
+        if constexpr (REF == 1) {
+            int chF[M], chB[M];
+            child_chains(job, chF, chB);
             int v = lb1p_child(d, job, chF, chB);
             return todo ? v : 0;
         }
@@ -557,7 +558,7 @@ struct Explorer {
         while (m) {
             int c = __ffsll((unsigned long long)m) - 1;
             m &= m - 1;
-            int v = lb2_child(c, d, chF, chB, rowmask, k);
+            int v = lb2_child(c, d, rowmask, k);
             if (lane == c) cs = v;
         }
         return cs;
@@ -568,9 +569,9 @@ struct Explorer {
     {
         const int k = n - L;
         jb += 2u * (unsigned)k;
-        int chF[M], chB[M];
-        child_chains(job, chF, chB);
-        if (a.refine == 0) {
+        if constexpr (REF == 1) {
+            int chF[M], chB[M];
+            child_chains(job, chF, chB);
             csF = valid ? lb1p_child(0, job, chF, chB) : 0;
             csB = valid ? lb1p_child(1, job, chF, chB) : 0;
             return;
@@ -578,8 +579,8 @@ struct Explorer {
         uint64_t rowmask = row_jobs(valid, job);
         csF = csB = 0;
         for (int c = 0; c < k; c++) {
-            int v0 = lb2_child(c, 0, chF, chB, rowmask, k);
-            int v1 = lb2_child(c, 1, chF, chB, rowmask, k);
+            int v0 = lb2_child(c, 0, rowmask, k);
+            int v1 = lb2_child(c, 1, rowmask, k);
             if (lane == c) {
                 csF = v0;
                 csB = v1;
@@ -778,6 +779,15 @@ struct Explorer {
     }
 
     // --------------------------------------------------- work sharing
+    // idle-wait pause: 2^k x 512 cycles, k growing to 4 while nothing
+    // happens, so thousands of idle wavefronts do not flood the memory
+    // system with polls
+    __device__ __forceinline__ static void idle_pause(int &nap)
+    {
+        for (int i = 0; i < nap; i++) __builtin_amdgcn_s_sleep(8);
+        if (nap < 16) nap <<= 1;
+    }
+
     __device__ __forceinline__ void fail(int code)
     {
         if (lane == 0) {

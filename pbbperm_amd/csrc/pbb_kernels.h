@@ -15,15 +15,20 @@ enum { PBB_DEV_ERR_RING = 1 };
 
 #define PBB_SPIN_LIMIT (1 << 24)
 
-// Device-resident control block of one engine.
+// Device-resident control block of one engine (or pool slot).  The ring
+// tickets and the work counter take atomics from every donor; the flags are
+// polled by idle explorers; each group sits on its own 128-byte line so the
+// polls do not queue behind the atomics.
 struct Ctl {
     unsigned idle_head;        // ring tickets (dequeue side)
     unsigned idle_tail;        // ring tickets (enqueue side)
+    int pad0[30];
     int work_count;            // explorers holding an interval
+    int pad1[31];
     int done;                  // set when work_count reaches 0
     int stop;                  // budget / deadline / interrupt / error
     int err;                   // device error code
-    int pad[2];
+    int pad2[29];
     unsigned long long dec;    // decomposed nodes
     unsigned long long leaves;
     unsigned long long sb;     // simple (LB1) bounds

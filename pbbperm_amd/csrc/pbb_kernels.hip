@@ -13,8 +13,9 @@
 //                  bbthread::shareWork (multicore/src/bbthread.cpp:39-72) and
 //                  hand the upper part to an idle explorer.  No host round
 //                  trip until the run ends (all idle, node budget, deadline
-//                  or interrupt).  REF=false compiles the LB1-only path
-//                  (boundingMode 0), REF=true adds the refinement bounds.
+//                  or interrupt).  REF selects the bounding variant at
+//                  compile time: 0 LB1 only (boundingMode 0), 1 LB1 + LB1'
+//                  refinement (modes 1/2 as shipped), 2 LB1 + Johnson LB2.
 //   k_bound_root<M> LB1 root bounding, ivm_bound::boundRoot (ivm_bound.cpp:273-330).
 //   k_bounds<M>    batched bound_abstract calls over device buffers, one
 //                  partial schedule per wavefront, on the explorer's own
@@ -27,11 +28,11 @@
 namespace pbb {
 
 // ------------------------------------------------------------ kernels
-template <int M, bool REF>
+template <int M, int REF>
 __global__ __launch_bounds__(256) void k_explore(KArgs a)
 {
     extern __shared__ __align__(16) char smem[];
-    Lds lds = lds_setup(a.I, smem, REF && a.refine == 1);
+    Lds lds = lds_setup(a.I, smem, REF == 2);
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= a.E) return;
 
@@ -53,6 +54,7 @@ __global__ __launch_bounds__(256) void k_explore(KArgs a)
     while (true) {
         if (st == ST_IDLE) {
             // wait for a donation, global termination or a stop
+            int nap = 1;
             while (true) {
                 int s = uni(ld_acq(a.state + e));
                 if (s == ST_ACTIVE) {
@@ -65,7 +67,7 @@ __global__ __launch_bounds__(256) void k_explore(KArgs a)
                     X.flush_counters();
                     return;
                 }
-                __builtin_amdgcn_s_sleep(8);
+                X.idle_pause(nap);
             }
         }
         // idle-ring poll issued before the node's work so its latency is hidden
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(64) void k_bound_root(KArgs a, const int *root_perm
 {
     extern __shared__ __align__(16) char smem[];
     Lds lds = lds_setup(a.I, smem, false);
-    Explorer<M, false> X(a, lds, 0, engine_inst(a), a.ctl, a.I.n, a.I.mp);
+    Explorer<M, 0> X(a, lds, 0, engine_inst(a), a.ctl, a.I.n, a.I.mp);
     X.local_best = a.initial_ub;
     X.rowv = X.lane < X.n ? root_perm[X.lane] : 0;
     X.reset_stacks();
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(64) void k_bound_root(KArgs a, const int *root_perm
 // (scheduleBack), remain <- the middle.  Then the explorer's own routines
 // produce the bounds.  Requires (l1+1) + (n-l2) < n.
 template <int M>
-__device__ __forceinline__ void load_partial(Explorer<M, true> &X, const int *perm, int l1, int l2)
+__device__ __forceinline__ void load_partial(Explorer<M, 2> &X, const int *perm, int l1, int l2)
 {
     X.reset_stacks();
     for (int i = 0; i <= l1; i++) X.push_job(uni(perm[i]), 0);
@@ -116,7 +118,7 @@ __device__ __forceinline__ void load_partial(Explorer<M, true> &X, const int *pe
 // makespan of perm with perm[mid] the only job between the stacks
 // (front = perm[0..mid-1], back = perm[mid+1..n-1])
 template <int M>
-__device__ __forceinline__ int makespan_mid(Explorer<M, true> &X, const int *perm, int mid)
+__device__ __forceinline__ int makespan_mid(Explorer<M, 2> &X, const int *perm, int mid)
 {
     load_partial<M>(X, perm, mid - 1, mid + 1);
     return X.makespan_with(uni(perm[mid]));
@@ -131,7 +133,7 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
     const int w = threadIdx.x >> 6;
     const int s = blockIdx.x * 4 + w;
     if (s >= count) return;
-    Explorer<M, true> X(a, lds, 0, engine_inst(a), a.ctl, a.I.n, a.I.mp);
+    Explorer<M, 2> X(a, lds, 0, engine_inst(a), a.ctl, a.I.n, a.I.mp);
     const int n = X.n, lane = X.lane;
     const int *perm = perms + (size_t)s * n;
     // inputs must be permutations of 0..n-1 with -1 <= l1 < l2 <= n;
@@ -236,17 +238,24 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
 
 // ---------------------------------------------------------- host launchers
 template <int M>
+static const void *explore_kernel(const KArgs &a)
+{
+    if (a.bound_mode == 0) return (const void *)k_explore<M, 0>;
+    return a.refine == 1 ? (const void *)k_explore<M, 2> : (const void *)k_explore<M, 1>;
+}
+
+template <int M>
 hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s)
 {
     const bool lb2 = a.bound_mode != 0 && a.refine == 1;
     const size_t lds = explore_lds_bytes(a.I, lb2);
-    if (a.bound_mode == 0) {
-        hipFuncSetAttribute((const void *)k_explore<M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((k_explore<M, false>), dim3(blocks), dim3(256), lds, s, a);
-    } else {
-        hipFuncSetAttribute((const void *)k_explore<M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL((k_explore<M, true>), dim3(blocks), dim3(256), lds, s, a);
-    }
+    hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (a.bound_mode == 0)
+        hipLaunchKernelGGL((k_explore<M, 0>), dim3(blocks), dim3(256), lds, s, a);
+    else if (a.refine == 1)
+        hipLaunchKernelGGL((k_explore<M, 2>), dim3(blocks), dim3(256), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_explore<M, 1>), dim3(blocks), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
@@ -259,12 +268,12 @@ hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
         *blocks_per_cu = 0;
         return hipSuccess;
     }
-    if (a.bound_mode == 0) {
-        hipFuncSetAttribute((const void *)k_explore<M, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, false>, 256, lds);
-    }
-    hipFuncSetAttribute((const void *)k_explore<M, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, true>, 256, lds);
+    hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (a.bound_mode == 0)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 0>, 256, lds);
+    if (a.refine == 1)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 2>, 256, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 1>, 256, lds);
 }
 
 template <int M>

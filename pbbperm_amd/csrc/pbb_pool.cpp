@@ -597,7 +597,7 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
     if (!cfg || !base || !best || count < 1 || !ptms || !n || !m || !ub)
         return fail(PBB_E_ARG, "bad autotune arguments");
     const int Ws[] = {1, 2, 4};
-    const int epis[] = {32, 256, 2048};
+    const int epis[] = {64, 256, 1024};
     const int margins[] = {2, 4};
     std::vector<pbb_result> res(count);
     double best_ms = 1e300;
@@ -616,8 +616,10 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
             for (int mg : margins) {
                 rc = pbb_pool_retune(p, epi, mg);
                 double ms = 0.0;
-                if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, 0.0, res.data(), &ms);  // warm
-                if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, 0.0, res.data(), &ms);
+                // a trial may not run longer than twice the best so far (+0.5 s)
+                const double cap = best_ms < 1e299 ? 2.0 * best_ms / 1e3 + 0.5 : 120.0;
+                if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, cap, res.data(), &ms);  // warm
+                if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, cap, res.data(), &ms);
                 if (rc) {
                     pbb_pool_destroy(p);
                     return rc;

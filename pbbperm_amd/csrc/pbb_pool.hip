@@ -275,7 +275,7 @@ static __device__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_e
 // ------------------------------------------------------------ retirement
 // the explorer whose release emptied the slot: record the optimum and its
 // schedule while every member is still inside, then mark the slot solved
-template <int M, bool REF>
+template <int M, int REF>
 __device__ __forceinline__ void pool_finish(Explorer<M, REF> &X, const KArgs &a, PoolSlot *S)
 {
     const unsigned long long b =
@@ -324,7 +324,7 @@ static __device__ void pool_leave(const PoolArgs &p, int s, int W)
 }
 
 // ----------------------------------------------------------------- kernel
-template <int M, bool REF>
+template <int M, int REF>
 __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
 {
     extern __shared__ __align__(16) char smem[];
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
     const int wave = threadIdx.x >> 6;
     const int e = blockIdx.x * W + wave;
     const int lane = lane_id();
-    const bool lb2 = REF && a.refine == 1;
+    const bool lb2 = REF == 2;
     unsigned long long t_end = 0ull;
     if (a.deadline) {
         unsigned long long t0 = wall_clock64();
@@ -401,6 +401,7 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
             for (;;) {
                 if (st == ST_IDLE) {
                     bool leave = false;
+                    int nap = 1;
                     while (true) {
                         const int sv = uni(ld_acq(a.state + e));
                         if (sv == ST_ACTIVE) {
@@ -413,7 +414,7 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
                             leave = true;
                             break;
                         }
-                        __builtin_amdgcn_s_sleep(8);
+                        X.idle_pause(nap);
                     }
                     if (leave) break;
                 }
@@ -467,7 +468,8 @@ __global__ void k_pool_init(PoolArgs p, int count)
 template <int M>
 static const void *pool_kernel(const KArgs &a)
 {
-    return a.bound_mode == 0 ? (const void *)k_pool<M, false> : (const void *)k_pool<M, true>;
+    if (a.bound_mode == 0) return (const void *)k_pool<M, 0>;
+    return a.refine == 1 ? (const void *)k_pool<M, 2> : (const void *)k_pool<M, 1>;
 }
 
 template <int M>
@@ -480,9 +482,11 @@ hipError_t launch_pool_m(const KArgs &a, const PoolArgs &p, int groups, int init
     if (e != hipSuccess) return e;
     hipFuncSetAttribute(pool_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (a.bound_mode == 0)
-        hipLaunchKernelGGL((k_pool<M, false>), dim3(groups), dim3(64 * p.W), lds, s, a, p);
+        hipLaunchKernelGGL((k_pool<M, 0>), dim3(groups), dim3(64 * p.W), lds, s, a, p);
+    else if (a.refine == 1)
+        hipLaunchKernelGGL((k_pool<M, 2>), dim3(groups), dim3(64 * p.W), lds, s, a, p);
     else
-        hipLaunchKernelGGL((k_pool<M, true>), dim3(groups), dim3(64 * p.W), lds, s, a, p);
+        hipLaunchKernelGGL((k_pool<M, 1>), dim3(groups), dim3(64 * p.W), lds, s, a, p);
     return hipGetLastError();
 }
 
@@ -496,8 +500,10 @@ hipError_t pool_occupancy_m(const KArgs &a, const PoolArgs &p, int *groups_per_c
     }
     hipFuncSetAttribute(pool_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (a.bound_mode == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, false>, 64 * p.W, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, true>, 64 * p.W, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, 0>, 64 * p.W, lds);
+    if (a.refine == 1)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, 2>, 64 * p.W, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, 1>, 64 * p.W, lds);
 }
 
 #ifdef PBB_M

@@ -10,6 +10,8 @@ the C ABI of ``libpbb_engine.so``.
 from __future__ import annotations
 
 import ctypes as C
+import json
+import os
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence, Tuple
 
@@ -183,7 +185,7 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
 
 
 GRID_W = (1, 2, 4)
-GRID_EPI = (32, 256, 2048)
+GRID_EPI = (64, 256, 1024)
 GRID_MARGIN = (2, 4)
 
 
@@ -191,7 +193,8 @@ def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Co
              base: Optional[PoolParams] = None, time_budget: float = 120.0, progress=None):
     """Pick waves per workgroup (block size), explorers per instance and the
     steal margin that solve the given instances fastest: every candidate of
-    GRID_W x GRID_EPI x GRID_MARGIN is timed on one warm graph launch.
+    GRID_W x GRID_EPI x GRID_MARGIN is timed on one warm graph launch, each
+    capped at twice the best time so far plus 0.5 s (a capped trial fails).
     Returns (params, trials) with trials rows (W, explorers_per_instance,
     margin, ms).  progress(row) is called after each trial."""
     cfg = config if config is not None else default_config()
@@ -210,8 +213,11 @@ def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Co
                     continue
                 for mg in GRID_MARGIN:
                     pool.retune(epi, mg)
-                    pool.solve(ptms, ubs, time_budget=time_budget)
-                    res, ms = pool.solve(ptms, ubs, time_budget=time_budget)
+                    # a trial may not run longer than twice the best so far (+0.5 s)
+                    done = [t[3] for t in trials if np.isfinite(t[3])]
+                    cap = min(time_budget, 2.0 * min(done) / 1e3 + 0.5) if done else time_budget
+                    pool.solve(ptms, ubs, time_budget=cap)
+                    res, ms = pool.solve(ptms, ubs, time_budget=cap)
                     if not all(r.status == "solved" for r in res):
                         ms = float("inf")
                     row = (W, epi, mg, ms)
@@ -283,3 +289,55 @@ def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
     gaps = rng.exponential(1.0 / rate, size=count)
     t = np.cumsum(gaps)
     return t - t[0]
+
+
+# ------------------------------------------------------ named workloads
+# Bounding / branching of both multi-instance workloads: LB1 for every child,
+# the Johnson LB2 over all machine pairs to refine the children kept in the
+# chosen direction (boundingMode 2, the commented-out refinement of
+# multicore/src/ivm_bound.cpp:171), MinMin branching (the reference's default
+# adaptiveBranchingMode 3).
+_LB12 = dict(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_johnson=0)
+
+WORKLOADS = {
+    "batched": dict(config=_LB12, max_jobs=22, machines=10, slots=512,
+                    tune=lambda: batch_stream(128, 1100)),
+    "serving": dict(config=_LB12, max_jobs=30, machines=20, slots=256,
+                    tune=lambda: serving_stream(64, 30000)),
+}
+
+TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
+
+
+def workload_config(name: str, **overrides) -> Config:
+    kw = dict(WORKLOADS[name]["config"])
+    kw.update(overrides)
+    return default_config(**kw)
+
+
+def workload_params(name: str, **overrides) -> PoolParams:
+    w = WORKLOADS[name]
+    kw = dict(max_jobs=w["max_jobs"], machines=w["machines"], slots=w["slots"])
+    kw.update(overrides)
+    return default_params(**kw)
+
+
+def tune_workload(name: str, progress=None):
+    """Run the autotuner on the workload's tuning sample (disjoint seeds from
+    the measured stream); returns (params, trials)."""
+    from .engine import neh
+    sample = WORKLOADS[name]["tune"]()
+    ubs = [neh(p)[0] for p in sample]
+    return autotune(sample, ubs, config=workload_config(name), base=workload_params(name), progress=progress)
+
+
+def tuned_params(name: str, path: str = TUNED_PATH) -> Optional[PoolParams]:
+    """The autotuner's stored choice for a workload, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f).get(name)
+    if not d:
+        return None
+    return workload_params(name, waves_per_group=d["waves_per_group"],
+                           explorers_per_instance=d["explorers_per_instance"], steal_margin=d["steal_margin"])
