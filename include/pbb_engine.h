@@ -184,6 +184,49 @@ int pbb_get_stats(pbb_engine *e, pbb_stats *st);
  * running kernel within a few thousand nodes. */
 int pbb_interrupt(pbb_engine *e);
 
+/* ------------------------------------------------ work-unit codec
+ * Factoradic digit vectors (n digits, digit i in [0, n-1-i], the fact_work
+ * pos/end layout) <-> decimal interval ranks sum_i digit_i * (n-1-i)!
+ * (fact_work::VectToBigint / BigintToVect, distributed/src/fact_work.cpp:47-86,
+ * weights in distributed/src/weights.cpp:10-26; fixed 384-bit arithmetic in
+ * place of GMP, ranks up to 63! - 1). */
+int pbb_fact_to_dec(int n, const int *digits, char *dec, int cap);
+int pbb_dec_to_fact(int n, const char *dec, int *digits);
+
+/* The checkpoint text of works::save (distributed/src/works.cpp:172-191):
+ * root solution, simpleBounds, incumbent (solution operator<<,
+ * common/src/solution.cpp:132-142), then the work units (works operator<<,
+ * distributed/src/works.cpp:558-569; work::writeHeader/writeIntervals,
+ * distributed/src/work.cpp:544-583: "id end_updated max_intervals
+ * exploredNodes nb" then "id begin end" per interval, then the total size). */
+typedef struct pbb_checkpoint {
+    int n;
+    int root_cost;
+    int root_perm[PBB_MAX_JOBS];
+    uint64_t simple_bounds;
+    int best_cost;
+    int best_perm[PBB_MAX_JOBS];
+    int work_id;
+    int max_intervals;
+    uint64_t explored;      /* exploredNodes (sum over units when parsing) */
+    int nb;                 /* intervals */
+} pbb_checkpoint;
+
+/* Text of one work unit holding intervals k < c->nb (ids, pos/end digit
+ * vectors); *len = text length; buf may be NULL to query it. */
+int pbb_checkpoint_format(const pbb_checkpoint *c, const int *ids, const int *pos, const int *end,
+                          char *buf, int cap, int *len);
+/* Parse a checkpoint text for n-job instances: every unit's non-empty
+ * intervals (begin <= end) into ids/pos/end (at most max). */
+int pbb_checkpoint_parse(const char *text, int n, pbb_checkpoint *c, int *ids, int *pos, int *end, int max);
+/* works::save of an engine: its active intervals, incumbent, root order,
+ * decomposed nodes and simple bounds, written to path. */
+int pbb_checkpoint_save(pbb_engine *e, const char *path);
+/* works::init_infile for an engine: root order, incumbent and intervals
+ * from path (interval i goes to explorer i); info (may be NULL) receives
+ * the file's counters. */
+int pbb_checkpoint_load(pbb_engine *e, const char *path, pbb_checkpoint *info);
+
 /* Explorer capacity the engine would use for (n, m, cfg) on this device. */
 int pbb_query_explorers(int n, int m, const pbb_config *cfg, int *explorers);
 

@@ -74,6 +74,16 @@ class Result(C.Structure):
     ]
 
 
+class Checkpoint(C.Structure):
+    """pbb_checkpoint: the header fields of a works::save checkpoint."""
+
+    _fields_ = [
+        ("n", C.c_int), ("root_cost", C.c_int), ("root_perm", C.c_int * PBB_MAX_JOBS),
+        ("simple_bounds", C.c_uint64), ("best_cost", C.c_int), ("best_perm", C.c_int * PBB_MAX_JOBS),
+        ("work_id", C.c_int), ("max_intervals", C.c_int), ("explored", C.c_uint64), ("nb", C.c_int),
+    ]
+
+
 IP = C.POINTER(C.c_int)
 I32P = C.POINTER(C.c_int32)
 U64P = C.POINTER(C.c_uint64)
@@ -126,6 +136,12 @@ SIGNATURES = {
                                    C.POINTER(C.c_double)]),
     "pbb_pool_autotune": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.c_int, I32P, IP, IP, IP,
                                     C.POINTER(PoolParams), C.POINTER(C.c_double), C.c_int, IP]),
+    "pbb_fact_to_dec": (C.c_int, [C.c_int, IP, C.c_char_p, C.c_int]),
+    "pbb_dec_to_fact": (C.c_int, [C.c_int, C.c_char_p, IP]),
+    "pbb_checkpoint_format": (C.c_int, [C.POINTER(Checkpoint), IP, IP, IP, C.c_char_p, C.c_int, IP]),
+    "pbb_checkpoint_parse": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(Checkpoint), IP, IP, IP, C.c_int]),
+    "pbb_checkpoint_save": (C.c_int, [VP, C.c_char_p]),
+    "pbb_checkpoint_load": (C.c_int, [VP, C.c_char_p, C.POINTER(Checkpoint)]),
     "pbb_last_error": (C.c_char_p, []),
     "pbb_version": (C.c_char_p, []),
 }

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cctype>
+#include <chrono>
 #include <climits>
 #include <cstdarg>
 #include <cstdio>
@@ -15,7 +16,6 @@
 #include <sstream>
 #include <string>
 #include <thread>
-#include <chrono>
 #include <vector>
 
 #include "../../include/pbb_engine.h"
@@ -845,11 +845,9 @@ extern "C" int pbb_run(pbb_engine *e, uint64_t node_budget, double time_budget_s
     return PBB_OK;
 }
 
-extern "C" int pbb_get_intervals(pbb_engine *e, int *pos, int *end, int *ids, int *nb, int max,
-                                 uint64_t *decomposed, uint64_t *leaves)
+// every active explorer's interval (state != idle)
+static int export_intervals(pbb_engine *e, int *pos, int *end, int *ids, int *nb, int max)
 {
-    if (!e || !pos || !end || !ids || !nb) return fail(PBB_E_ARG, "null argument");
-    if (max < e->E) return fail(PBB_E_CAPACITY, "buffer too small (%d < %d)", max, e->E);
     const int n = e->n, E = e->E;
     std::vector<int> st(E);
     std::vector<int8_t> hp((size_t)E * 64), he((size_t)E * 64);
@@ -869,6 +867,16 @@ extern "C" int pbb_get_intervals(pbb_engine *e, int *pos, int *end, int *ids, in
             k++;
         }
     *nb = k;
+    return PBB_OK;
+}
+
+extern "C" int pbb_get_intervals(pbb_engine *e, int *pos, int *end, int *ids, int *nb, int max,
+                                 uint64_t *decomposed, uint64_t *leaves)
+{
+    if (!e || !pos || !end || !ids || !nb) return fail(PBB_E_ARG, "null argument");
+    if (max < e->E) return fail(PBB_E_CAPACITY, "buffer too small (%d < %d)", max, e->E);
+    int rc = export_intervals(e, pos, end, ids, nb, max);
+    if (rc) return rc;
     Ctl c;
     HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
     if (decomposed) *decomposed = c.dec - e->exported_dec;
@@ -1039,4 +1047,64 @@ extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t 
     cleanup();
     if (wall_s) *wall_s = count ? t_last - arrival_s[0] : 0.0;
     return rc;
+}
+
+// ----------------------------------------------------------- checkpoint
+extern "C" int pbb_checkpoint_save(pbb_engine *e, const char *path)
+{
+    if (!e || !path) return fail(PBB_E_ARG, "null argument");
+    if (!e->initialized) return fail(PBB_E_STATE, "checkpoint before pbb_init_full/pbb_init_intervals");
+    const int n = e->n, E = e->E;
+    std::vector<int> pos((size_t)E * n), end((size_t)E * n), ids(E);
+    int nb = 0;
+    int rc = export_intervals(e, pos.data(), end.data(), ids.data(), &nb, E);
+    if (rc) return rc;
+    if ((rc = sync_incumbent(e))) return rc;
+    pbb_stats st;
+    if ((rc = pbb_get_stats(e, &st))) return rc;
+    pbb_checkpoint c;
+    memset(&c, 0, sizeof c);
+    c.n = n;
+    c.root_cost = e->initial_ub;
+    for (int i = 0; i < n; i++) {
+        c.root_perm[i] = e->root_perm[i];
+        c.best_perm[i] = e->inc_perm[i];
+    }
+    c.simple_bounds = st.simple_bounds;
+    c.best_cost = e->inc_cost;
+    c.work_id = 0;
+    c.max_intervals = E;
+    c.explored = st.decomposed;
+    c.nb = nb;
+    int len = 0;
+    if ((rc = pbb_checkpoint_format(&c, ids.data(), pos.data(), end.data(), nullptr, 0, &len))) return rc;
+    std::vector<char> buf(len + 1);
+    if ((rc = pbb_checkpoint_format(&c, ids.data(), pos.data(), end.data(), buf.data(), len + 1, &len))) return rc;
+    FILE *f = fopen(path, "w");
+    if (!f) return fail(PBB_E_ARG, "cannot write %s", path);
+    size_t w = fwrite(buf.data(), 1, len, f);
+    fclose(f);
+    if (w != (size_t)len) return fail(PBB_E_ARG, "short write to %s", path);
+    return PBB_OK;
+}
+
+extern "C" int pbb_checkpoint_load(pbb_engine *e, const char *path, pbb_checkpoint *info)
+{
+    if (!e || !path) return fail(PBB_E_ARG, "null argument");
+    std::ifstream f(path);
+    if (!f) return fail(PBB_E_ARG, "cannot read %s", path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    const int n = e->n, E = e->E;
+    std::vector<int> pos((size_t)E * n), end((size_t)E * n), ids(E);
+    pbb_checkpoint c;
+    int rc = pbb_checkpoint_parse(text.c_str(), n, &c, ids.data(), pos.data(), end.data(), E);
+    if (rc) return rc;
+    if ((rc = pbb_set_root(e, c.root_perm))) return rc;
+    if ((rc = pbb_set_incumbent(e, c.best_cost, c.best_perm))) return rc;
+    for (int k = 0; k < c.nb; k++) ids[k] = k;  // interval k -> explorer k
+    if ((rc = pbb_init_intervals(e, c.nb, ids.data(), pos.data(), end.data()))) return rc;
+    if (info) *info = c;
+    return PBB_OK;
 }

@@ -21,7 +21,7 @@ from typing import List, Optional, Sequence, Tuple
 import numpy as np
 
 from . import _lib
-from ._lib import Config, PbbError, Stats, check, load
+from ._lib import Checkpoint, Config, PbbError, Stats, check, load
 
 INT_MAX = 2**31 - 1
 
@@ -287,10 +287,72 @@ class GpuBB:
         return SolveStats(st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds, st.explorers,
                           st.active, st.kernel_ms, list(st.depth_histo)[: self.n + 1])
 
+    def save_checkpoint(self, path: str):
+        """works::save: active intervals, incumbent, root order, counters."""
+        check(load().pbb_checkpoint_save(self._h, path.encode()))
+
+    def load_checkpoint(self, path: str) -> dict:
+        """works::init_infile: restore root order, incumbent and intervals."""
+        c = Checkpoint()
+        check(load().pbb_checkpoint_load(self._h, path.encode(), C.byref(c)))
+        return {"explored": c.explored, "simple_bounds": c.simple_bounds, "best_cost": c.best_cost, "nb": c.nb}
+
     def solve(self, node_budget: int = 0, time_budget: float = 0.0) -> bool:
         """init_full_interval + next until done (or budget). Returns True when finished."""
         self.init_full_interval()
         return self.next(node_budget, time_budget)
+
+
+# ------------------------------------------------------- work-unit codec
+
+def fact_to_dec(digits: Sequence[int]) -> str:
+    """Decimal rank of a factoradic digit vector (fact_work::VectToBigint)."""
+    d = np.ascontiguousarray(digits, dtype=np.intc)
+    buf = C.create_string_buffer(128)
+    check(load().pbb_fact_to_dec(len(d), _ip(d), buf, 128))
+    return buf.value.decode()
+
+
+def dec_to_fact(n: int, dec) -> np.ndarray:
+    """Factoradic digits of a decimal rank (fact_work::BigintToVect)."""
+    d = np.zeros(n, dtype=np.intc)
+    check(load().pbb_dec_to_fact(n, str(dec).encode(), _ip(d)))
+    return d
+
+
+def checkpoint_format(n: int, ids, pos, end, root_cost: int, root_perm, simple_bounds: int, best_cost: int,
+                      best_perm, explored: int = 0, work_id: int = 0, max_intervals: int = 0) -> str:
+    """works::save text of one work unit holding the given intervals."""
+    c = Checkpoint()
+    c.n, c.root_cost, c.simple_bounds, c.best_cost = n, root_cost, simple_bounds, best_cost
+    c.work_id, c.max_intervals, c.explored = work_id, max_intervals or max(1, len(ids)), explored
+    for i in range(n):
+        c.root_perm[i] = int(root_perm[i])
+        c.best_perm[i] = int(best_perm[i])
+    ids = np.ascontiguousarray(ids, dtype=np.intc)
+    pos = np.ascontiguousarray(pos, dtype=np.intc).reshape(-1)
+    end = np.ascontiguousarray(end, dtype=np.intc).reshape(-1)
+    c.nb = len(ids)
+    ln = C.c_int()
+    check(load().pbb_checkpoint_format(C.byref(c), _ip(ids), _ip(pos), _ip(end), None, 0, C.byref(ln)))
+    buf = C.create_string_buffer(ln.value + 1)
+    check(load().pbb_checkpoint_format(C.byref(c), _ip(ids), _ip(pos), _ip(end), buf, ln.value + 1, C.byref(ln)))
+    return buf.value.decode()
+
+
+def checkpoint_parse(text: str, n: int, max_intervals: int = 1 << 16):
+    """-> (header dict, ids, pos[nb, n], end[nb, n]) of a works::save text."""
+    c = Checkpoint()
+    ids = np.zeros(max_intervals, dtype=np.intc)
+    pos = np.zeros(max_intervals * n, dtype=np.intc)
+    end = np.zeros(max_intervals * n, dtype=np.intc)
+    check(load().pbb_checkpoint_parse(text.encode(), n, C.byref(c), _ip(ids), _ip(pos), _ip(end), max_intervals))
+    hdr = {"root_cost": c.root_cost, "root_perm": [c.root_perm[i] for i in range(n)],
+           "simple_bounds": c.simple_bounds, "best_cost": c.best_cost,
+           "best_perm": [c.best_perm[i] for i in range(n)], "work_id": c.work_id,
+           "max_intervals": c.max_intervals, "explored": c.explored, "nb": c.nb}
+    k = c.nb
+    return hdr, ids[:k].copy(), pos[:k * n].reshape(k, n).copy(), end[:k * n].reshape(k, n).copy()
 
 
 def explorers_for(n: int, m: int, config: Optional[Config] = None) -> int:

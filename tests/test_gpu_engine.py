@@ -233,6 +233,35 @@ def test_export_import_intervals(oracle_mod):
     eng2.close()
 
 
+def test_checkpoint_save_load_resume(oracle_mod, tmp_path):
+    """works::save after a budgeted run, works::init_infile into a fresh
+    engine (incumbent and intervals from the file), run to the end: the
+    explored count in the file plus the resumed run's count equals the
+    oracle's single-thread count, and the optimum is the same."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    for (seed, n, m, mode) in [(42, 16, 8, (0, 1, 0, 0)), (3, 16, 6, (2, 3, 1, 0))]:
+        ptm = O.synth_ptm(seed, n, m)
+        opt = _opt(O, ptm)
+        bm, br, rf, jp = mode
+        ref = O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+        eng = P.GpuBB(ptm, _cfg(P, bm, br, rf, jp, nb=256), ub=opt)
+        eng.init_full_interval()
+        assert not eng.next(node_budget=max(1000, ref.decomposed // 5), time_budget=60.0)
+        path = str(tmp_path / f"bab{seed}.save")
+        eng.save_checkpoint(path)
+        first = eng.stats()
+        eng.close()
+        eng2 = P.GpuBB(ptm, _cfg(P, bm, br, rf, jp, nb=512))
+        info = eng2.load_checkpoint(path)
+        assert info["explored"] == first.decomposed and info["best_cost"] == opt
+        assert info["simple_bounds"] == first.simple_bounds
+        assert eng2.next(time_budget=60.0)
+        assert info["explored"] + eng2.stats().decomposed == ref.decomposed
+        assert eng2.best()[0] == opt
+        eng2.close()
+
+
 def test_interrupt_and_continue(oracle_mod):
     import time
     import pbbperm_amd as P
