@@ -277,6 +277,17 @@ int pbb_pool_info(pbb_pool *p, int *explorers, int *groups, int *lds_bytes);
  * (the cap stays within the idle-ring capacity chosen at creation). */
 int pbb_pool_retune(pbb_pool *p, int explorers_per_instance, int steal_margin);
 
+/* A request's ub may be PBB_UB_NEH: the admitting workgroup then computes
+ * the incumbent with NEH on the device (pbb_neh_device semantics) and the
+ * result's schedule is the NEH order unless the search improves on it. */
+#define PBB_UB_NEH -1
+
+/* NEH for count instances on the device, one wavefront each (the
+ * pbb_neh order with identity start; instance i's times at ptms + off_i as
+ * in pbb_pool_solve).  perms: count x 64 ints, instance i's order at
+ * perms[64 i .. 64 i + n[i] - 1]; costs: count makespans. */
+int pbb_neh_device(int count, const int32_t *ptms, const int *n, const int *m, int *perms, int *costs);
+
 /* Batch: count <= slots instances, instance i's m[i] x n[i] machine-major
  * times at ptms[off_i], off_i = sum of n[k]*m[k] for k < i.  Upload, reset,
  * the resident launch and the result download are one graph launch.

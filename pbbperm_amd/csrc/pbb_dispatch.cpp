@@ -71,4 +71,12 @@ hipError_t pool_occupancy(int m, const KArgs &a, const PoolArgs &p, int *groups_
 #undef C_
 }
 
+hipError_t launch_neh(int m, const int *ptms, const long long *off, const int *ns, const int *ms, int count,
+                      int nmax, int *perms, int *costs, hipStream_t s)
+{
+#define C_(M) launch_neh_m<M>(ptms, off, ns, ms, count, nmax, perms, costs, s)
+    PBB_FOR_M(C_, m)
+#undef C_
+}
+
 }  // namespace pbb

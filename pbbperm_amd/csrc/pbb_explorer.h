@@ -152,6 +152,64 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
     return L;
 }
 
+// NEH (heuristic/src/fastNEH.cpp:151-243, with the intended partial
+// makespan at every insertion position): one wavefront.  Jobs get stable
+// ranks by decreasing total time; then job k is inserted into the first k
+// at the first position of least partial makespan (lane = position, each
+// lane evaluates its candidate sequence on M machines).  The first-minimum
+// rule also settles the first two jobs as fastNEH.cpp:233-236 does (the
+// original order only when strictly better).  ptmT: [n][mp] job-major;
+// seq: n ints of the wave's LDS, receives the order.  Returns the makespan.
+template <int M>
+__device__ __forceinline__ int neh_wave(const int *ptmT, int mp, int n, int *seq)
+{
+    const int lane = lane_id();
+    int tot = 0;
+    if (lane < n)
+#pragma unroll
+        for (int q = 0; q < M; q++) tot += ptmT[lane * mp + q];
+    int rank = 0;
+    for (int i = 0; i < n; i++) {
+        const int ti = rdlane(tot, i);
+        rank += (ti > tot || (ti == tot && i < lane)) ? 1 : 0;
+    }
+    if (lane < n) seq[rank] = lane;
+    __builtin_amdgcn_wave_barrier();
+    int cmin = 0;
+    if (n == 1) {
+        int c = 0;
+#pragma unroll
+        for (int q = 0; q < M; q++) c += ptmT[seq[0] * mp + q];
+        return c;
+    }
+    for (int k = 1; k < n; k++) {
+        const int job = seq[k];
+        int cost = 0x7fffffff;
+        if (lane <= k) {
+            int cv[M];
+#pragma unroll
+            for (int q = 0; q < M; q++) cv[q] = 0;
+            for (int i = 0; i <= k; i++) {
+                const int j = i < lane ? seq[i] : (i == lane ? job : seq[i - 1]);
+                const int *pj = ptmT + j * mp;
+                cv[0] += pj[0];
+#pragma unroll
+                for (int q = 1; q < M; q++) cv[q] = max(cv[q - 1], cv[q]) + pj[q];
+            }
+            cost = cv[M - 1];
+        }
+        cmin = wave_min(cost);
+        const int pb = __ffsll((unsigned long long)ballot(lane <= k && cost == cmin)) - 1;
+        const int v = lane < n ? seq[lane] : 0;
+        const int vprev = shfl(v, lane > 0 ? lane - 1 : 0);
+        __builtin_amdgcn_wave_barrier();
+        if (lane == pb) seq[lane] = job;
+        else if (lane > pb && lane <= k) seq[lane] = vprev;
+        __builtin_amdgcn_wave_barrier();
+    }
+    return cmin;
+}
+
 // instance reference of the one-instance engine: everything from the
 // kernel arguments
 __device__ __forceinline__ InstRef engine_inst(const KArgs &a)

@@ -222,6 +222,9 @@ hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perm
 template <int M> hipError_t launch_pool_m(const KArgs &a, const PoolArgs &p, int groups, int init_count,
                                           hipStream_t s);
 template <int M> hipError_t pool_occupancy_m(const KArgs &a, const PoolArgs &p, int *groups_per_cu);
+template <int M>
+hipError_t launch_neh_m(const int *ptms, const long long *off, const int *ns, const int *ms, int count, int nmax,
+                        int *perms, int *costs, hipStream_t s);
 
 // runtime-M dispatch (pbb_dispatch.cpp)
 hipError_t launch_explore(const KArgs &a, int blocks, hipStream_t s);
@@ -233,5 +236,7 @@ hipError_t launch_bounds(int which, const KArgs &a, int count, const int *perms,
 // init_count > 0: batch launch, the queue holds slots 0..init_count-1
 hipError_t launch_pool(int M, const KArgs &a, const PoolArgs &p, int groups, int init_count, hipStream_t s);
 hipError_t pool_occupancy(int M, const KArgs &a, const PoolArgs &p, int *groups_per_cu);
+hipError_t launch_neh(int M, const int *ptms, const long long *off, const int *ns, const int *ms, int count,
+                      int nmax, int *perms, int *costs, hipStream_t s);
 
 }  // namespace pbb

@@ -646,3 +646,53 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
     if (ntrials) *ntrials = nt;
     return PBB_OK;
 }
+
+// ------------------------------------------------------------ device NEH
+extern "C" int pbb_neh_device(int count, const int32_t *ptms, const int *n, const int *m, int *perms, int *costs)
+{
+    if (count < 0 || (count > 0 && (!ptms || !n || !m || !perms || !costs))) return fail(PBB_E_ARG, "bad arguments");
+    if (count == 0) return PBB_OK;
+    int rc = check_device();
+    if (rc) return rc;
+    std::vector<long long> off(count);
+    long long tot = 0;
+    int M = 2, nmax = 1;
+    for (int i = 0; i < count; i++) {
+        if ((rc = check_shape(ptms + tot, n[i], m[i]))) return rc;
+        off[i] = tot;
+        tot += (long long)n[i] * m[i];
+        M = std::max(M, m[i]);
+        nmax = std::max(nmax, n[i]);
+    }
+    int32_t *dp = nullptr;
+    long long *doff = nullptr;
+    int *dn = nullptr, *dm = nullptr, *dperm = nullptr, *dcost = nullptr;
+    auto cleanup = [&]() {
+        hipFree(dp);
+        hipFree(doff);
+        hipFree(dn);
+        hipFree(dm);
+        hipFree(dperm);
+        hipFree(dcost);
+    };
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc(&dp, (size_t)tot * 4));
+        HIPCHK(hipMalloc(&doff, (size_t)count * 8));
+        HIPCHK(hipMalloc(&dn, (size_t)count * 4));
+        HIPCHK(hipMalloc(&dm, (size_t)count * 4));
+        HIPCHK(hipMalloc(&dperm, (size_t)count * 64 * 4));
+        HIPCHK(hipMalloc(&dcost, (size_t)count * 4));
+        HIPCHK(hipMemcpy(dp, ptms, (size_t)tot * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(doff, off.data(), (size_t)count * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dn, n, (size_t)count * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm, m, (size_t)count * 4, hipMemcpyHostToDevice));
+        HIPCHK(launch_neh(M, dp, doff, dn, dm, count, nmax, dperm, dcost, nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(perms, dperm, (size_t)count * 64 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(costs, dcost, (size_t)count * 4, hipMemcpyDeviceToHost));
+        return PBB_OK;
+    };
+    rc = run();
+    cleanup();
+    return rc;
+}

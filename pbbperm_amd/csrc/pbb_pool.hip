@@ -286,7 +286,7 @@ __device__ __forceinline__ void pool_finish(Explorer<M, REF> &X, const KArgs &a,
     if (imp && X.lane < X.n) S->perm[X.lane] = a.best_perm[(size_t)who * 64 + X.lane];
     if (X.lane == 0) {
         S->best_cost = cost;
-        S->improved = imp ? 1 : 0;
+        if (imp) S->improved = 1;  // else the schedule given at admission (NEH), if any, stands
         S->status = RES_SOLVED;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -366,9 +366,24 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
                                       __HIP_MEMORY_SCOPE_AGENT);
         } else {
             const int n = uni(ld_rlx(&S->n)), m = uni(ld_rlx(&S->m)), npairs = uni(ld_rlx(&S->npairs));
-            const int ub = uni(ld_rlx(&S->ub));
             const DevInst I = slot_inst<M>(p, slot, n, npairs);
             const Lds lds = lds_setup(I, smem, p.jt_lds != 0);
+            if (role == ROLE_OWNER) {
+                // a request without an incumbent gets NEH's (ub < 0)
+                if (wave == 0 && uni(ld_rlx(&S->ub)) < 0) {
+                    const int c = neh_wave<M>(lds.ptmT, I.mp, n, lds.scr);
+                    if (lane < n) S->perm[lane] = lds.scr[lane];
+                    if (lane == 0) {
+                        S->ub = c;
+                        S->best_cost = c;
+                        S->improved = 1;
+                        S->best = ((unsigned long long)(unsigned)c << 32) | 0xffffffffull;
+                    }
+                    __threadfence();
+                }
+                __syncthreads();
+            }
+            const int ub = uni(ld_rlx(&S->ub));
             Explorer<M, REF> X(a, lds, e, slot_ref(p, slot, S, m, npairs, ub), p.run, n, I.mp);
             X.t_end = t_end;
             X.local_best = uni((int)(unsigned)(ld_rlx(&S->best) >> 32));
@@ -464,6 +479,30 @@ __global__ void k_pool_init(PoolArgs p, int count)
     }
 }
 
+// NEH for many instances, one wavefront each (times [m][n] at ptms + off[i],
+// padded to M machines with zero times, which changes no partial makespan)
+template <int M>
+__global__ __launch_bounds__(64) void k_neh(const int *ptms, const long long *off, const int *ns, const int *ms,
+                                            int count, int nmax, int *perms, int *costs)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int s = blockIdx.x;
+    if (s >= count) return;
+    const int n = ns[s], m = ms[s], lane = lane_id();
+    const int mp = (M + 3) & ~3;
+    int *ptmT = (int *)smem;
+    int *seq = ptmT + nmax * mp;
+    const int *src = ptms + off[s];
+    for (int i = lane; i < n * mp; i += PBB_WAVE) {
+        const int j = i / mp, q = i - j * mp;
+        ptmT[i] = q < m ? src[q * n + j] : 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int c = neh_wave<M>(ptmT, mp, n, seq);
+    if (lane < n) perms[(size_t)s * 64 + lane] = seq[lane];
+    if (lane == 0) costs[s] = c;
+}
+
 // ---------------------------------------------------------- host launchers
 template <int M>
 static const void *pool_kernel(const KArgs &a)
@@ -506,7 +545,18 @@ hipError_t pool_occupancy_m(const KArgs &a, const PoolArgs &p, int *groups_per_c
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, 1>, 64 * p.W, lds);
 }
 
+template <int M>
+hipError_t launch_neh_m(const int *ptms, const long long *off, const int *ns, const int *ms, int count, int nmax,
+                        int *perms, int *costs, hipStream_t s)
+{
+    const size_t lds = (size_t)nmax * ((M + 3) & ~3) * 4 + 64 * 4;
+    hipLaunchKernelGGL(k_neh<M>, dim3(count), dim3(64), lds, s, ptms, off, ns, ms, count, nmax, perms, costs);
+    return hipGetLastError();
+}
+
 #ifdef PBB_M
+template hipError_t launch_neh_m<PBB_M>(const int *, const long long *, const int *, const int *, int, int, int *,
+                                        int *, hipStream_t);
 template hipError_t launch_pool_m<PBB_M>(const KArgs &, const PoolArgs &, int, int, hipStream_t);
 template hipError_t pool_occupancy_m<PBB_M>(const KArgs &, const PoolArgs &, int *);
 #endif

@@ -169,6 +169,19 @@ class Pool:
         return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
 
 
+UB_NEH = -1  # a request's ub: compute NEH on the device at admission
+
+
+def neh_device(ptms: Sequence[np.ndarray]):
+    """NEH of many instances on the device -> (orders list, costs array)."""
+    flat, ns, ms = _pack(ptms)
+    perms = np.zeros(64 * max(1, len(ptms)), dtype=np.intc)
+    costs = np.zeros(max(1, len(ptms)), dtype=np.intc)
+    check(load().pbb_neh_device(len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms), _ip(perms),
+                                _ip(costs)))
+    return [perms[64 * i:64 * i + int(ns[i])].copy() for i in range(len(ptms))], costs[:len(ptms)].copy()
+
+
 def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Sequence[float],
                  config: Optional[Config] = None, time_budget: float = 0.0):
     """The same stream on the one-instance engine, one launch per instance, FCFS."""

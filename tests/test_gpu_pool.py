@@ -287,3 +287,36 @@ def test_autotuner_picks_a_valid_configuration(oracle_mod):
     with PP.Pool(_cfg(P, 2, 3, 1, 0), params=best) as pool:
         res, _ = pool.solve(ptms, opts, time_budget=BUDGET)
     assert all(r.status == "solved" and r.best == u for r, u in zip(res, opts))
+
+
+def test_device_neh_matches_host_neh():
+    """NEH on the device (one wavefront per instance) gives the host NEH's
+    order and makespan exactly, from 3 to 63 jobs and 2 to 20 machines."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    shapes = [(3, 2), (5, 3), (12, 5), (20, 5), (20, 10), (30, 20), (50, 20), (63, 20), (63, 7), (17, 13)]
+    ptms = [P.synth(900 + i, n, m) for i, (n, m) in enumerate(shapes)]
+    ptms += PP.serving_stream(120, 40000)
+    orders, costs = PP.neh_device(ptms)
+    for p, o, c in zip(ptms, orders, costs):
+        hc, hp = P.neh(p)
+        assert int(c) == hc
+        assert list(o) == list(hp)
+        assert P.makespan(p, o) == hc
+
+
+def test_pool_computes_neh_at_admission(oracle_mod):
+    """Requests with ub = UB_NEH: the pool's NEH incumbent then the search;
+    the optimum is exact and the returned schedule realises it."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    O = oracle_mod
+    ptms = _small_batch(O, 40, 7600, 9, 13)
+    opts = [_opt(O, p) for p in ptms]
+    with PP.Pool(_cfg(P, 2, 3, 1, 0), max_jobs=13, machines=5, slots=40, waves_per_group=4,
+                 explorers_per_instance=64) as pool:
+        res, _ = pool.solve(ptms, [PP.UB_NEH] * len(ptms), time_budget=BUDGET)
+    for p, r, u in zip(ptms, res, opts):
+        assert r.status == "solved" and r.best == u
+        assert r.improved
+        _check_schedule(P, p, r)
