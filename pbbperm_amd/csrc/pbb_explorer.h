@@ -107,43 +107,45 @@ __device__ __forceinline__ int chain_scan(int base, int p)
 
 // ------------------------------------------------------------ LDS layout
 struct Lds {
-    int *ptmT;      // [n][mp] job-major (uniform per-job rows)
-    int *pm;        // [m][n] machine-major (per-lane loads of different jobs: distinct banks)
+    int *pm;        // [m][ps] machine-major processing times
+    int ps;         // row stride of pm (odd)
     int *minA, *minD, *tot;
     int4 *jtab;     // Johnson table (LDS or global)
-    int8_t *mat;    // this wave's IVM matrix, row r at mat + r*64
+    int8_t *mat;    // this wave's IVM matrix, row l at mat + lds_mat_row(l, n)
     int *stk;       // this wave's stacks: F entry s at row s, B entry s at row n-s ((n+1) x m)
     int2 *U;        // this wave's LB1 operand block [2][m]
-    int *scr;       // this wave's scratch (64 ints)
+    int *scr;       // this wave's scratch (64 ints; LB2 variant only)
 };
 
-__device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load_jtab)
+__device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load_jtab, bool scratch)
 {
     Lds L;
     const int n = I.n, mp = I.mp, m = I.m;
-    L.ptmT = (int *)smem;
-    L.pm = L.ptmT + n * mp;
-    L.minA = L.pm + n * mp;
+    L.ps = lds_pm_stride(n);
+    L.pm = (int *)smem;
+    L.minA = L.pm + m * L.ps;
     L.minD = L.minA + mp;
     L.tot = L.minD + mp;
-    size_t off = (size_t)(2 * n + 3) * mp * 4;
+    size_t off = ((size_t)m * L.ps + 3 * (size_t)mp) * 4;
     off = (off + 15) & ~(size_t)15;
     load_jtab = load_jtab && jtab_in_lds(n, I.npairs);
     L.jtab = load_jtab ? (int4 *)(smem + off) : const_cast<int4 *>(I.jtab);
-    char *wbase = smem + lds_table_bytes(n, mp, I.npairs, load_jtab) +
-                  (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n, m);
+    char *wbase = smem + lds_table_bytes(n, m, I.npairs, load_jtab) +
+                  (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n, m, scratch);
     L.mat = (int8_t *)wbase;
-    L.stk = (int *)(wbase + (size_t)n * 64);
-    L.U = (int2 *)(wbase + (size_t)n * 64 + lds_stack_bytes(n, m));
-    L.scr = (int *)(wbase + (size_t)n * 64 + lds_stack_bytes(n, m) + (size_t)m * 16);
-    for (int i = threadIdx.x; i < n * mp; i += blockDim.x) L.ptmT[i] = I.ptmT[i];
-    for (int i = threadIdx.x; i < n * m; i += blockDim.x) L.pm[i] = I.ptm[i];
+    L.stk = (int *)(wbase + lds_mat_bytes(n));
+    L.U = (int2 *)(wbase + lds_mat_bytes(n) + lds_stack_bytes(n, m));
+    L.scr = scratch ? (int *)(wbase + lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16) : nullptr;
+    for (int i = threadIdx.x; i < n * m; i += blockDim.x) {
+        const int q = i / n, j = i - q * n;
+        L.pm[q * L.ps + j] = I.ptm[i];
+    }
     for (int i = threadIdx.x; i < mp; i += blockDim.x) {
         L.minA[i] = i < m ? I.min_arr[i] : 0;
         L.minD[i] = i < m ? I.min_dep[i] : 0;
         int s = 0;
         if (i < m)
-            for (int j = 0; j < n; j++) s += I.ptmT[j * mp + i];
+            for (int j = 0; j < n; j++) s += I.ptm[i * n + j];
         L.tot[i] = s;
     }
     if (load_jtab)
@@ -158,16 +160,16 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
 // at the first position of least partial makespan (lane = position, each
 // lane evaluates its candidate sequence on M machines).  The first-minimum
 // rule also settles the first two jobs as fastNEH.cpp:233-236 does (the
-// original order only when strictly better).  ptmT: [n][mp] job-major;
+// original order only when strictly better).  pm: [M][ps] machine-major;
 // seq: n ints of the wave's LDS, receives the order.  Returns the makespan.
 template <int M>
-__device__ __forceinline__ int neh_wave(const int *ptmT, int mp, int n, int *seq)
+__device__ __forceinline__ int neh_wave(const int *pm, int ps, int n, int *seq)
 {
     const int lane = lane_id();
     int tot = 0;
     if (lane < n)
 #pragma unroll
-        for (int q = 0; q < M; q++) tot += ptmT[lane * mp + q];
+        for (int q = 0; q < M; q++) tot += pm[q * ps + lane];
     int rank = 0;
     for (int i = 0; i < n; i++) {
         const int ti = rdlane(tot, i);
@@ -179,7 +181,7 @@ __device__ __forceinline__ int neh_wave(const int *ptmT, int mp, int n, int *seq
     if (n == 1) {
         int c = 0;
 #pragma unroll
-        for (int q = 0; q < M; q++) c += ptmT[seq[0] * mp + q];
+        for (int q = 0; q < M; q++) c += pm[q * ps + seq[0]];
         return c;
     }
     for (int k = 1; k < n; k++) {
@@ -191,10 +193,9 @@ __device__ __forceinline__ int neh_wave(const int *ptmT, int mp, int n, int *seq
             for (int q = 0; q < M; q++) cv[q] = 0;
             for (int i = 0; i <= k; i++) {
                 const int j = i < lane ? seq[i] : (i == lane ? job : seq[i - 1]);
-                const int *pj = ptmT + j * mp;
-                cv[0] += pj[0];
+                cv[0] += pm[j];
 #pragma unroll
-                for (int q = 1; q < M; q++) cv[q] = max(cv[q - 1], cv[q]) + pj[q];
+                for (int q = 1; q < M; q++) cv[q] = max(cv[q - 1], cv[q]) + pm[q * ps + j];
             }
             cost = cv[M - 1];
         }
@@ -283,7 +284,7 @@ struct Explorer {
 
     __device__ __forceinline__ int pmach(int job) const  // lane q: p[q][job]
     {
-        return lane < M ? lds.ptmT[job * mp + lane] : 0;
+        return lane < M ? lds.pm[lane * lds.ps + job] : 0;
     }
 
     __device__ __forceinline__ void reset_stacks()
@@ -310,7 +311,7 @@ struct Explorer {
         } else {
             // back chain runs from machine M-1 down: lane q handles machine M-1-q
             int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
-            int pr = lane < M ? lds.ptmT[j * mp + (M - 1 - lane)] : 0;
+            int pr = lane < M ? lds.pm[(M - 1 - lane) * lds.ps + j] : 0;
             int c = chain_scan(base, pr);
             if (lane < M) brow(nb + 1)[M - 1 - lane] = c;
             nb++;
@@ -371,8 +372,8 @@ struct Explorer {
         const int job = base == 0 ? bcast_low_half(rowv) : bcast_high_half(rowv);
         const int jb = c < k ? job : 0;
         // machine-major rows: front lanes walk machines 0..M-1, back lanes M-1..0
-        const int *pcol = lds.pm + (half ? (M - 1) * n : 0) + jb;
-        const int pstep = half ? -n : n;
+        const int *pcol = lds.pm + (half ? (M - 1) * lds.ps : 0) + jb;
+        const int pstep = half ? -lds.ps : lds.ps;
         const int2 *u = lds.U + half * M;
         int pj[M];
 #pragma unroll
@@ -508,18 +509,19 @@ struct Explorer {
     // d = 0: front' = front + job; d = 1: back' = job + back.
     __device__ __forceinline__ void child_chains(int job, int (&chF)[M], int (&chB)[M]) const
     {
-        const int *pj = lds.ptmT + job * mp;
+        const int *pj = lds.pm + job;  // machine q at pj[q * ps]
+        const int ps = lds.ps;
         int t = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
             int f = fval(q);
-            t = (q == 0 ? f : max(f, t)) + pj[q];
+            t = (q == 0 ? f : max(f, t)) + pj[q * ps];
             chF[q] = t;
         }
 #pragma unroll
         for (int q = M - 1; q >= 0; q--) {
             int b = bval(q);
-            t = (q == M - 1 ? b : max(b, t)) + pj[q];
+            t = (q == M - 1 ? b : max(b, t)) + pj[q * ps];
             chB[q] = t;
         }
     }
@@ -528,13 +530,14 @@ struct Explorer {
     // child's front or back)
     __device__ __forceinline__ int lb1p_child(int d, int job, const int (&chF)[M], const int (&chB)[M]) const
     {
-        const int *pj = lds.ptmT + job * mp;
+        const int *pj = lds.pm + job;  // machine q at pj[q * ps]
+        const int ps = lds.ps;
         int t0 = 0, lb = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
             int f = d == 0 ? chF[q] : fval(q);
             int b = d == 1 ? chB[q] : bval(q);
-            int r = rval(q) - pj[q];
+            int r = rval(q) - pj[q * ps];
             if (q == 0) {
                 t0 = f + r;
                 lb = t0 + b;
@@ -566,7 +569,7 @@ struct Explorer {
         } else {
             // back chain over machines M-1..0: lane q handles machine M-1-q
             const int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
-            const int pr = lane < M ? lds.ptmT[cj * mp + (M - 1 - lane)] : 0;
+            const int pr = lane < M ? lds.pm[(M - 1 - lane) * lds.ps + cj] : 0;
             const int cb = chain_scan(base, pr);
             if (lane < M) {
                 fs[lane] = frow(nf)[lane];
@@ -676,7 +679,7 @@ struct Explorer {
     __device__ __forceinline__ void write_perm(int mid)
     {
         int jl = 0;
-        if (lane < L) jl = flag_abs((int)lds.mat[lane * 64 + posv]);
+        if (lane < L) jl = flag_abs((int)lds.mat[lds_mat_row(lane, n) + posv]);
         uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         uint64_t fr = ballot(lane < L && dirv == 0);
         uint64_t bk = ballot(lane < L && dirv != 0);
@@ -691,7 +694,7 @@ struct Explorer {
     // ------------------------------------------------------- navigation
     __device__ __forceinline__ void store_row(int line) const
     {
-        if (lane < n - line) lds.mat[line * 64 + lane] = (int8_t)rowv;
+        if (lane < n - line) lds.mat[lds_mat_row(line, n) + lane] = (int8_t)rowv;
     }
 
     __device__ __forceinline__ void go_up()
@@ -699,7 +702,7 @@ struct Explorer {
         if (L > 0) {
             posv = wrlane(posv, 0, L);
             L--;
-            rowv = lane < n - L ? (int)lds.mat[L * 64 + lane] : 0;
+            rowv = lane < n - L ? (int)lds.mat[lds_mat_row(L, n) + lane] : 0;
             const int p = rdlane(posv, L);
             pop_job(flag_abs(rdlane(rowv, p)), rdlane(dirv, L));
             if (lane == p) rowv = flag_neg(rowv);
@@ -773,7 +776,7 @@ struct Explorer {
         reset_stacks();
         for (int l = 0; l < L; l++) {
             int p = rdlane(posv, l);
-            int j = uni(flag_abs((int)lds.mat[l * 64 + p]));
+            int j = uni(flag_abs((int)lds.mat[lds_mat_row(l, n) + p]));
             push_job(j, rdlane(dirv, l));
         }
     }
@@ -783,7 +786,7 @@ struct Explorer {
         L = uni(a.line[e]);
         const int8_t *gm = a.mat + (size_t)e * rows * 64;
         for (int r = 0; r < L; r++)
-            if (lane < n) lds.mat[r * 64 + lane] = gm[r * 64 + lane];
+            if (lane < n - r) lds.mat[lds_mat_row(r, n) + lane] = gm[r * 64 + lane];
         rowv = lane < n - L ? (int)gm[L * 64 + lane] : 0;
         posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
         endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
@@ -796,7 +799,7 @@ struct Explorer {
     {
         int8_t *gm = a.mat + (size_t)e * rows * 64;
         for (int r = 0; r < L; r++)
-            if (lane < n) gm[r * 64 + lane] = lds.mat[r * 64 + lane];
+            if (lane < n - r) gm[r * 64 + lane] = lds.mat[lds_mat_row(r, n) + lane];
         if (lane < n) gm[L * 64 + lane] = (int8_t)rowv;
         if (lane < n) {
             a.pos[(size_t)e * 64 + lane] = (int8_t)posv;
@@ -881,7 +884,7 @@ struct Explorer {
             pl = rdlane(posv, l);
             el = (l == first) ? rdlane(endv, l) : n - 1 - l;
             if (pl < el) {
-                rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[l * 64 + lane] : -1);
+                rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[lds_mat_row(l, n) + lane] : -1);
                 expl = ballot(lane > pl && lane <= el && rowl >= 0);
                 if (expl) break;
             }
@@ -925,7 +928,7 @@ struct Explorer {
         }
         int8_t *gm = a.mat + (size_t)thief * rows * 64;
         for (int r = 0; r < l; r++)
-            if (lane < n) gm[r * 64 + lane] = lds.mat[r * 64 + lane];
+            if (lane < n - r) gm[r * 64 + lane] = lds.mat[lds_mat_row(r, n) + lane];
         if (lane < n) gm[l * 64 + lane] = (int8_t)rowl;
         if (lane < n) {
             a.pos[(size_t)thief * 64 + lane] = (int8_t)(lane < l ? posv : (lane == l ? cut : 0));

@@ -84,33 +84,46 @@ struct KArgs {
     int steal_margin;
 };
 
-// LDS footprint.  Workgroup tables: job-major processing times ptmT[n][mp],
-// the machine-major copy pm[m][n] (stored in an n*mp slot), min_arr, min_dep, per-machine
-// totals, and the Johnson table npairs*n int4 when LB2 is used and it fits.
-// Per wavefront: the IVM matrix (n rows of 64 bytes), the front/back stack
-// ((n+1) x m ints), the LB1 operand block (2 x m int2) and 64 ints of scratch.
-__device__ __host__ inline size_t lds_table_bytes(int n, int mp, int npairs, bool lb2)
+// LDS footprint.  Workgroup tables: the machine-major processing times
+// pm[m][ps] with an odd row stride ps = n | 1 (lanes reading one job on
+// different machines, or different jobs on one machine, hit distinct banks),
+// min_arr, min_dep, per-machine totals, and the Johnson table npairs*n int4
+// when LB2 is used and it fits.  Per wavefront: the IVM matrix (row l holds
+// its n-l cells, rows packed), the front/back stack ((n+1) x m ints), the
+// LB1 operand block (2 x m int2) and, for the LB2 variant, 64 ints of
+// scratch.
+__device__ __host__ inline int lds_pm_stride(int n) { return n | 1; }
+__device__ __host__ inline size_t lds_table_bytes(int n, int m, int npairs, bool lb2)
 {
-    size_t off = (size_t)(2 * n + 3) * mp * 4;
+    const int mp = (m + 3) & ~3;
+    size_t off = ((size_t)m * lds_pm_stride(n) + 3 * (size_t)mp) * 4;
     off = (off + 15) & ~(size_t)15;
     if (lb2) off += (size_t)npairs * n * 16;
     return off;
 }
-__device__ __host__ inline size_t lds_stack_bytes(int n, int m) { return ((size_t)(n + 1) * m * 4 + 15) & ~(size_t)15; }
-__device__ __host__ inline size_t lds_wave_bytes(int n, int m)
+// offset of IVM row l (rows 0..l-1 hold n, n-1, ..., n-l+1 cells)
+__device__ __host__ inline int lds_mat_row(int l, int n) { return l * n - (l * (l - 1)) / 2; }
+__device__ __host__ inline size_t lds_mat_bytes(int n)
 {
-    return (size_t)n * 64 + lds_stack_bytes(n, m) + (size_t)m * 16 + 64 * 4;
+    size_t t = (size_t)n * (n + 1) / 2;
+    if (t < (size_t)4 * n) t = (size_t)4 * n;  // also holds n ints (NEH order at admission)
+    return (t + 15) & ~(size_t)15;
+}
+__device__ __host__ inline size_t lds_stack_bytes(int n, int m) { return ((size_t)(n + 1) * m * 4 + 15) & ~(size_t)15; }
+__device__ __host__ inline size_t lds_wave_bytes(int n, int m, bool scratch)
+{
+    return lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16 + (scratch ? 64 * 4 : 0);
 }
 // The Johnson table is staged in LDS when it is at most 64 KB (m <= 10 at
 // n <= 50, m = 20 at n <= 21); larger tables are read from global memory.
 __device__ __host__ inline bool jtab_in_lds(int n, int npairs) { return (size_t)npairs * n * 16 <= 64 * 1024; }
 inline size_t explore_lds_bytes(const DevInst &I, bool lb2)
 {
-    return lds_table_bytes(I.n, I.mp, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) + 4 * lds_wave_bytes(I.n, I.m);
+    return lds_table_bytes(I.n, I.m, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) + 4 * lds_wave_bytes(I.n, I.m, lb2);
 }
 inline size_t root_lds_bytes(const DevInst &I)
 {
-    return lds_table_bytes(I.n, I.mp, I.npairs, false) + lds_wave_bytes(I.n, I.m);
+    return lds_table_bytes(I.n, I.m, I.npairs, false) + lds_wave_bytes(I.n, I.m, false);
 }
 
 // ------------------------------------------------------------------ pool
@@ -207,7 +220,7 @@ struct PoolArgs {
 
 __device__ __host__ inline size_t pool_lds_bytes(int nmax, int M, int pmax, bool jt_lds, int W)
 {
-    return lds_table_bytes(nmax, (M + 3) & ~3, pmax, jt_lds) + (size_t)W * lds_wave_bytes(nmax, M);
+    return lds_table_bytes(nmax, M, pmax, jt_lds) + (size_t)W * lds_wave_bytes(nmax, M, pmax > 0);
 }
 
 // per-M kernels (pbb_kernels.hip, one translation unit per M)

@@ -28,11 +28,16 @@
 namespace pbb {
 
 // ------------------------------------------------------------ kernels
+// Minimum resident waves per SIMD the compiler must allow (register budget).
+#ifndef PBB_EXPLORE_MIN_WAVES
+#define PBB_EXPLORE_MIN_WAVES 1
+#endif
+
 template <int M, int REF>
-__global__ __launch_bounds__(256) void k_explore(KArgs a)
+__global__ __launch_bounds__(256, PBB_EXPLORE_MIN_WAVES) void k_explore(KArgs a)
 {
     extern __shared__ __align__(16) char smem[];
-    Lds lds = lds_setup(a.I, smem, REF == 2);
+    Lds lds = lds_setup(a.I, smem, REF == 2, REF == 2);
     const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (e >= a.E) return;
 
@@ -91,7 +96,7 @@ template <int M>
 __global__ __launch_bounds__(64) void k_bound_root(KArgs a, const int *root_perm)
 {
     extern __shared__ __align__(16) char smem[];
-    Lds lds = lds_setup(a.I, smem, false);
+    Lds lds = lds_setup(a.I, smem, false, false);
     Explorer<M, 0> X(a, lds, 0, engine_inst(a), a.ctl, a.I.n, a.I.mp);
     X.local_best = a.initial_ub;
     X.rowv = X.lane < X.n ? root_perm[X.lane] : 0;
@@ -129,7 +134,7 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
                                                 const int *l1s, const int *l2s, int arg, int *o1, int *o2)
 {
     extern __shared__ __align__(16) char smem[];
-    Lds lds = lds_setup(a.I, smem, which == 2);
+    Lds lds = lds_setup(a.I, smem, which == 2, true);
     const int w = threadIdx.x >> 6;
     const int s = blockIdx.x * 4 + w;
     if (s >= count) return;
@@ -290,7 +295,7 @@ hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perm
                            const int *l2, int arg, int *o1, int *o2, hipStream_t s)
 {
     const bool lb2 = which == 2 && jtab_in_lds(a.I.n, a.I.npairs);
-    const size_t lds = lds_table_bytes(a.I.n, a.I.mp, a.I.npairs, lb2) + 4 * lds_wave_bytes(a.I.n, a.I.m);
+    const size_t lds = lds_table_bytes(a.I.n, a.I.m, a.I.npairs, lb2) + 4 * lds_wave_bytes(a.I.n, a.I.m, true);
     hipFuncSetAttribute((const void *)k_bounds<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_bounds<M>, dim3((count + 3) / 4), dim3(256), lds, s, a, which, count, perms, l1, l2,
                        arg, o1, o2);

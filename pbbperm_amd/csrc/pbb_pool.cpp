@@ -10,6 +10,7 @@
 #include <climits>
 #include <cstring>
 #include <deque>
+#include <iterator>
 #include <thread>
 #include <vector>
 
@@ -596,8 +597,9 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
 {
     if (!cfg || !base || !best || count < 1 || !ptms || !n || !m || !ub)
         return fail(PBB_E_ARG, "bad autotune arguments");
-    const int Ws[] = {1, 2, 4};
-    const int epis[] = {64, 256, 1024};
+    // most-promising first, so the time cap tightens early
+    const int Ws[] = {4, 2, 1};
+    const int epis[] = {256, 1024, 64};
     const int margins[] = {2, 4};
     std::vector<pbb_result> res(count);
     double best_ms = 1e300;
@@ -606,7 +608,7 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
     for (int W : Ws) {
         pbb_pool_params prm = *base;
         prm.waves_per_group = W;
-        prm.explorers_per_instance = epis[sizeof epis / sizeof epis[0] - 1];
+        prm.explorers_per_instance = *std::max_element(std::begin(epis), std::end(epis));
         prm.slots = std::max(base->slots, count);
         pbb_pool *p = nullptr;
         int rc = pbb_pool_create(cfg, &prm, &p);

@@ -367,12 +367,13 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
         } else {
             const int n = uni(ld_rlx(&S->n)), m = uni(ld_rlx(&S->m)), npairs = uni(ld_rlx(&S->npairs));
             const DevInst I = slot_inst<M>(p, slot, n, npairs);
-            const Lds lds = lds_setup(I, smem, p.jt_lds != 0);
+            const Lds lds = lds_setup(I, smem, p.jt_lds != 0, REF == 2);
             if (role == ROLE_OWNER) {
                 // a request without an incumbent gets NEH's (ub < 0)
                 if (wave == 0 && uni(ld_rlx(&S->ub)) < 0) {
-                    const int c = neh_wave<M>(lds.ptmT, I.mp, n, lds.scr);
-                    if (lane < n) S->perm[lane] = lds.scr[lane];
+                    int *seq = (int *)lds.mat;  // the IVM rows are not in use yet
+                    const int c = neh_wave<M>(lds.pm, lds.ps, n, seq);
+                    if (lane < n) S->perm[lane] = seq[lane];
                     if (lane == 0) {
                         S->ub = c;
                         S->best_cost = c;
@@ -489,16 +490,16 @@ __global__ __launch_bounds__(64) void k_neh(const int *ptms, const long long *of
     const int s = blockIdx.x;
     if (s >= count) return;
     const int n = ns[s], m = ms[s], lane = lane_id();
-    const int mp = (M + 3) & ~3;
-    int *ptmT = (int *)smem;
-    int *seq = ptmT + nmax * mp;
+    const int ps = lds_pm_stride(nmax);
+    int *pm = (int *)smem;
+    int *seq = pm + M * ps;
     const int *src = ptms + off[s];
-    for (int i = lane; i < n * mp; i += PBB_WAVE) {
-        const int j = i / mp, q = i - j * mp;
-        ptmT[i] = q < m ? src[q * n + j] : 0;
+    for (int i = lane; i < M * n; i += PBB_WAVE) {
+        const int q = i / n, j = i - q * n;
+        pm[q * ps + j] = q < m ? src[q * n + j] : 0;
     }
     __builtin_amdgcn_wave_barrier();
-    const int c = neh_wave<M>(ptmT, mp, n, seq);
+    const int c = neh_wave<M>(pm, ps, n, seq);
     if (lane < n) perms[(size_t)s * 64 + lane] = seq[lane];
     if (lane == 0) costs[s] = c;
 }
@@ -549,7 +550,7 @@ template <int M>
 hipError_t launch_neh_m(const int *ptms, const long long *off, const int *ns, const int *ms, int count, int nmax,
                         int *perms, int *costs, hipStream_t s)
 {
-    const size_t lds = (size_t)nmax * ((M + 3) & ~3) * 4 + 64 * 4;
+    const size_t lds = (size_t)M * lds_pm_stride(nmax) * 4 + 64 * 4;
     hipLaunchKernelGGL(k_neh<M>, dim3(count), dim3(64), lds, s, ptms, off, ns, ms, count, nmax, perms, costs);
     return hipGetLastError();
 }

@@ -197,8 +197,9 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
     return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
 
 
-GRID_W = (1, 2, 4)
-GRID_EPI = (64, 256, 1024)
+# searched most-promising first, so the time cap tightens early
+GRID_W = (4, 2, 1)
+GRID_EPI = (256, 1024, 64)
 GRID_MARGIN = (2, 4)
 
 
