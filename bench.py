@@ -46,7 +46,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per decomposed node of k_explore on this workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
 # plain 256 MiB device copy): profiles/k_explore_50x20_lb1_pmc.md
-HBM_BYTES_PER_NODE = 5.32
+HBM_BYTES_PER_NODE = 2.72
 
 
 def lb1_ops(depth: int, n: int, m: int) -> int:
@@ -298,7 +298,7 @@ def main():
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
                      "hbm_GBs": traffic / (kernel_ms / args.steps / 1e3) / 1e9,
                      "hbm_frac": traffic / (kernel_ms / args.steps / 1e3) / 1e9 / PEAK_HBM_GBS,
-                     "kernel": "k_explore<20,false>",
+                     "kernel": "k_explore<20,0>",
                      "kernel_ms_per_step": kernel_ms / args.steps,
                      "ops_per_step": ops / args.steps},
     }

@@ -322,7 +322,7 @@ int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, cons
 
 /* Autotuner: time pbb_pool_solve on the given instances for every
  * (waves_per_group in {1,2,4}, explorers_per_instance in {64,256,1024},
- * steal_margin in {2,4}) candidate (each trial capped at twice the best
+ * steal_margin in {2,4,8}) candidate (each trial capped at twice the best
  * time so far plus 0.5 s; capped trials count as failed) and
  * write the fastest into *best (other fields taken from *base).  trials
  * (may be NULL) receives up to max_trials rows {W, epi, margin, ms}. */

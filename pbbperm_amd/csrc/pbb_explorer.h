@@ -866,11 +866,12 @@ struct Explorer {
     // victim's interval covers exactly the same live nodes) and the split
     // moves down to the next line.  Node counts are unchanged by either step.
     // h, t: idle-ring head/tail loaded before the node's work (prefetch).
-    __device__ __forceinline__ void try_donate(unsigned h, unsigned t)
+    // Returns whether idle explorers were waiting (the ring was not empty).
+    __device__ __forceinline__ bool try_donate(unsigned h, unsigned t)
     {
         h = (unsigned)uni((int)h);
         t = (unsigned)uni((int)t);
-        if (h == t) return;
+        if (h == t) return false;
         int lim = min(L, n - a.steal_margin);
         if (lim < 0) lim = 0;
         uint64_t ne = ballot(lane < n && posv != endv);
@@ -880,7 +881,7 @@ struct Explorer {
         int l = first, pl = 0, el = 0, rowl = -1;
         uint64_t expl = 0ull;
         for (;; l++) {
-            if (l > lim) return;
+            if (l > lim) return true;
             pl = rdlane(posv, l);
             el = (l == first) ? rdlane(endv, l) : n - 1 - l;
             if (pl < el) {
@@ -896,7 +897,7 @@ struct Explorer {
             else if (lane >= l && lane < n) endE = n - 1 - lane;
         }
         int nexp = __popcll(expl);
-        if (!w_cas(&ir.ctl->idle_head, h, h + 1)) return;
+        if (!w_cas(&ir.ctl->idle_head, h, h + 1)) return true;
         // claim the idle explorer registered at ticket h (bounded MPMC ring cell)
         const unsigned slot = h & (unsigned)(ir.ring_cap - 1);
         bool ok = false;
@@ -909,13 +910,13 @@ struct Explorer {
         }
         if (!ok) {
             fail(PBB_DEV_ERR_RING);
-            return;
+            return true;
         }
         const int thief = uni(ld_rlx(ir.ring_id + slot)) - 1;
         if (lane == 0) st_rel((int *)(ir.ring_seq + slot), (int)(h + (unsigned)ir.ring_cap));
         if (thief < 0 || thief >= a.E) {
             fail(PBB_DEV_ERR_RING);
-            return;
+            return true;
         }
         int keep = nexp / 2;
         int cut;
@@ -942,6 +943,7 @@ struct Explorer {
         if (lane > l && lane < n) endv = n - 1 - lane;
         w_add(&ir.ctl->work_count, 1);
         if (lane == 0) st_rel(a.state + thief, ST_ACTIVE);
+        return true;
     }
 
     // enter the idle ring (cell sequence protocol of a bounded MPMC queue)

@@ -56,6 +56,7 @@ __global__ __launch_bounds__(256, PBB_EXPLORE_MIN_WAVES) void k_explore(KArgs a)
     }
 
     unsigned iter = 0;
+    bool hungry = true;
     while (true) {
         if (st == ST_IDLE) {
             // wait for a donation, global termination or a stop
@@ -76,8 +77,12 @@ __global__ __launch_bounds__(256, PBB_EXPLORE_MIN_WAVES) void k_explore(KArgs a)
             }
         }
         // idle-ring poll issued before the node's work so its latency is hidden
-        const unsigned rh = ld_rlx(&a.ctl->idle_head);
-        const unsigned rt = ld_rlx(&a.ctl->idle_tail);
+        const bool poll = hungry || (iter & (PBB_POLL_EVERY - 1)) == 0u;
+        unsigned rh = 0, rt = 0;
+        if (poll) {
+            rh = ld_rlx(&a.ctl->idle_head);
+            rt = ld_rlx(&a.ctl->idle_tail);
+        }
         if (!X.step()) {
             if (X.go_idle() && X.lane == 0) st_rel(&a.ctl->done, 1);
             st = ST_IDLE;
@@ -88,7 +93,7 @@ __global__ __launch_bounds__(256, PBB_EXPLORE_MIN_WAVES) void k_explore(KArgs a)
             X.flush_counters();
             return;
         }
-        X.try_donate(rh, rt);
+        if (poll) hungry = X.try_donate(rh, rt);
     }
 }
 

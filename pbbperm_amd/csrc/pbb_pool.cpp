@@ -600,7 +600,7 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
     // most-promising first, so the time cap tightens early
     const int Ws[] = {4, 2, 1};
     const int epis[] = {256, 1024, 64};
-    const int margins[] = {2, 4};
+    const int margins[] = {2, 4, 8};
     std::vector<pbb_result> res(count);
     double best_ms = 1e300;
     int nt = 0;
@@ -621,13 +621,18 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
                 // a trial may not run longer than twice the best so far (+0.5 s)
                 const double cap = best_ms < 1e299 ? 2.0 * best_ms / 1e3 + 0.5 : 120.0;
                 if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, cap, res.data(), &ms);  // warm
-                if (!rc) rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, cap, res.data(), &ms);
+                ms = 1e300;
+                for (int r = 0; r < 2 && !rc; r++) {  // best of two (the search order varies)
+                    double t = 0.0;
+                    rc = pbb_pool_solve(p, count, ptms, n, m, ub, nullptr, cap, res.data(), &t);
+                    bool all = true;
+                    for (int i = 0; i < count; i++) all = all && res[i].status == PBB_RES_SOLVED;
+                    if (!rc && all) ms = std::min(ms, t);
+                }
                 if (rc) {
                     pbb_pool_destroy(p);
                     return rc;
                 }
-                for (int i = 0; i < count; i++)
-                    if (res[i].status != PBB_RES_SOLVED) ms = 1e300;
                 if (trials && nt < max_trials) {
                     trials[4 * nt + 0] = W;
                     trials[4 * nt + 1] = epi;

@@ -414,6 +414,7 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
                 X.register_idle();
             }
             unsigned iter = 0;
+            bool hungry = true;
             for (;;) {
                 if (st == ST_IDLE) {
                     bool leave = false;
@@ -434,15 +435,19 @@ __global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
                     }
                     if (leave) break;
                 }
-                const unsigned rh = ld_rlx(&S->c.idle_head);
-                const unsigned rt = ld_rlx(&S->c.idle_tail);
+                const bool poll = hungry || (iter & (PBB_POLL_EVERY - 1)) == 0u;
+                unsigned rh = 0, rt = 0;
+                if (poll) {
+                    rh = ld_rlx(&S->c.idle_head);
+                    rt = ld_rlx(&S->c.idle_tail);
+                }
                 if (!X.step()) {
                     if (X.go_idle()) pool_finish(X, a, S);
                     st = ST_IDLE;
                     continue;
                 }
                 if ((++iter & 63u) == 0u && X.check_stop()) break;
-                X.try_donate(rh, rt);
+                if (poll) hungry = X.try_donate(rh, rt);
             }
             X.flush_counters();
         }

@@ -200,15 +200,16 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
 # searched most-promising first, so the time cap tightens early
 GRID_W = (4, 2, 1)
 GRID_EPI = (256, 1024, 64)
-GRID_MARGIN = (2, 4)
+GRID_MARGIN = (2, 4, 8)
 
 
 def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Config] = None,
              base: Optional[PoolParams] = None, time_budget: float = 120.0, progress=None):
     """Pick waves per workgroup (block size), explorers per instance and the
     steal margin that solve the given instances fastest: every candidate of
-    GRID_W x GRID_EPI x GRID_MARGIN is timed on one warm graph launch, each
-    capped at twice the best time so far plus 0.5 s (a capped trial fails).
+    GRID_W x GRID_EPI x GRID_MARGIN is timed as the best of two graph
+    launches after a warm one, each capped at twice the best time so far plus
+    0.5 s (a capped launch fails).
     Returns (params, trials) with trials rows (W, explorers_per_instance,
     margin, ms).  progress(row) is called after each trial."""
     cfg = config if config is not None else default_config()
@@ -231,9 +232,11 @@ def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Co
                     done = [t[3] for t in trials if np.isfinite(t[3])]
                     cap = min(time_budget, 2.0 * min(done) / 1e3 + 0.5) if done else time_budget
                     pool.solve(ptms, ubs, time_budget=cap)
-                    res, ms = pool.solve(ptms, ubs, time_budget=cap)
-                    if not all(r.status == "solved" for r in res):
-                        ms = float("inf")
+                    ms = float("inf")
+                    for _ in range(2):  # best of two timed launches (the search order varies)
+                        res, t = pool.solve(ptms, ubs, time_budget=cap)
+                        if all(r.status == "solved" for r in res):
+                            ms = min(ms, t)
                     row = (W, epi, mg, ms)
                     trials.append(row)
                     if progress:
@@ -315,7 +318,7 @@ _LB12 = dict(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_jo
 
 WORKLOADS = {
     "batched": dict(config=_LB12, max_jobs=22, machines=10, slots=512,
-                    tune=lambda: batch_stream(128, 1100)),
+                    tune=lambda: batch_stream(512, 1100)),
     "serving": dict(config=_LB12, max_jobs=30, machines=20, slots=256,
                     tune=lambda: serving_stream(64, 30000)),
 }
