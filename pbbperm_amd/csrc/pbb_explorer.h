@@ -716,7 +716,10 @@ struct Explorer {
     {
         store_row(L);
         push_job(flag_abs(rdlane(rowv, c)), rdlane(dirv, L));
-        int v = shfl(rowv, lane + (lane >= c ? 1 : 0));
+        // drop cell c: lanes >= c take their right neighbour (DPP wave_shl:1,
+        // lane i reads lane i+1; lane 63 reads 0)
+        const int nxt = __builtin_amdgcn_update_dpp(0, rowv, 0x130, 0xF, 0xF, false);
+        int v = lane >= c ? nxt : rowv;
         L++;
         rowv = lane < n - L ? flag_abs(v) : 0;
         if (explore) {

@@ -324,8 +324,13 @@ static __device__ void pool_leave(const PoolArgs &p, int s, int W)
 }
 
 // ----------------------------------------------------------------- kernel
+// Minimum resident waves per SIMD the compiler must allow (register budget).
+#ifndef PBB_POOL_MIN_WAVES
+#define PBB_POOL_MIN_WAVES 1
+#endif
+
 template <int M, int REF>
-__global__ __launch_bounds__(256) void k_pool(KArgs a, PoolArgs p)
+__global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolArgs p)
 {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_pick[2];
