@@ -180,6 +180,14 @@ int pbb_set_incumbent(pbb_engine *e, int cost, const int *perm);
 
 int pbb_get_stats(pbb_engine *e, pbb_stats *st);
 
+/* gpubb::getDeepSubproblem (gpu/src/gpubb.cu:1313-1371): the current
+ * schedule of every active explorer (front jobs, the current row in order,
+ * back jobs: ivm_bound::prepareSchedule, multicore/src/ivm_bound.cpp:111-138),
+ * lowest makespan first (ties by explorer), at most max of them, n ints each
+ * in perms; *count receives how many.  Used by the worker to seed its
+ * heuristic (distributed/src/worker_gpu.cpp:93-110). */
+int pbb_get_deep_subproblems(pbb_engine *e, int *perms, int max, int *count);
+
 /* gpubb::interruptExploration (gpu/src/gpubb.cu:156-162); honoured by the
  * running kernel within a few thousand nodes. */
 int pbb_interrupt(pbb_engine *e);

@@ -116,6 +116,7 @@ SIGNATURES = {
     "pbb_set_incumbent": (C.c_int, [VP, C.c_int, IP]),
     "pbb_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
     "pbb_interrupt": (C.c_int, [VP]),
+    "pbb_get_deep_subproblems": (C.c_int, [VP, IP, C.c_int, IP]),
     "pbb_query_explorers": (C.c_int, [C.c_int, C.c_int, C.POINTER(Config), IP]),
     "pbb_pool_params_default": (C.c_int, [C.POINTER(PoolParams)]),
     "pbb_pool_create": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.POINTER(VP)]),

@@ -273,9 +273,7 @@ extern "C" int pbb_instance_read_vrf(const char *path, int *n_out, int *m_out, i
 
 static int host_partial_makespan(const int32_t *ptm, int n, int m, const int *seq, int len)
 {
-    int c[PBB_MAX_MACHINES * 4] = {0};
     std::vector<int> cv(m, 0);
-    (void)c;
     for (int i = 0; i < len; i++) {
         int j = seq[i];
         cv[0] += ptm[j];
@@ -927,6 +925,50 @@ extern "C" int pbb_get_stats(pbb_engine *e, pbb_stats *st)
     st->active = act;
     st->kernel_ms = e->last_ms;
     for (int i = 0; i <= PBB_MAX_JOBS + 1; i++) st->depth_histo[i] = c.histo[i];
+    return PBB_OK;
+}
+
+extern "C" int pbb_get_deep_subproblems(pbb_engine *e, int *perms, int max, int *count)
+{
+    if (!e || !count || max < 0 || (max > 0 && !perms)) return fail(PBB_E_ARG, "bad arguments");
+    *count = 0;
+    if (!e->initialized || max == 0) return PBB_OK;
+    const int n = e->n, m = e->m, E = e->E;
+    std::vector<int> st(E), line(E);
+    std::vector<int8_t> pos((size_t)E * 64), dir((size_t)E * 64), mat((size_t)E * n * 64);
+    HIPCHK(hipMemcpy(st.data(), e->state, (size_t)E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(line.data(), e->line, (size_t)E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(pos.data(), e->pos, pos.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(dir.data(), e->dir, dir.size(), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(mat.data(), e->mat, mat.size(), hipMemcpyDeviceToHost));
+    std::vector<std::pair<int, int>> order;  // (cost, explorer)
+    std::vector<int> sched((size_t)E * n);
+    for (int i = 0; i < E; i++) {
+        if (st[i] != ST_ACTIVE) continue;
+        const int L = line[i];
+        if (L < 0 || L >= n) continue;
+        const int8_t *rows = mat.data() + (size_t)i * n * 64;
+        int *sc = sched.data() + (size_t)i * n;
+        int l1 = 0, l2 = n - 1;
+        for (int l = 0; l < L; l++) {
+            const int p = pos[(size_t)i * 64 + l];
+            const int v = rows[l * 64 + p];
+            const int job = v < 0 ? -v - 1 : v;
+            if (dir[(size_t)i * 64 + l] == 0) sc[l1++] = job;
+            else sc[l2--] = job;
+        }
+        for (int c = 0; c < n - L; c++) {
+            const int v = rows[L * 64 + c];
+            sc[l1 + c] = v < 0 ? -v - 1 : v;
+        }
+        order.emplace_back(host_partial_makespan(e->T.ptm.data(), n, m, sc, n), i);
+    }
+    std::sort(order.begin(), order.end());
+    const int k = std::min((int)order.size(), max);
+    for (int j = 0; j < k; j++)
+        std::copy(sched.begin() + (size_t)order[j].second * n, sched.begin() + (size_t)(order[j].second + 1) * n,
+                  perms + (size_t)j * n);
+    *count = k;
     return PBB_OK;
 }
 

@@ -287,6 +287,14 @@ class GpuBB:
         return SolveStats(st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds, st.explorers,
                           st.active, st.kernel_ms, list(st.depth_histo)[: self.n + 1])
 
+    def get_deep_subproblems(self, max_count: int = 64) -> np.ndarray:
+        """gpubb::getDeepSubproblem: current schedules of the active explorers,
+        lowest makespan first -> array [k, n]."""
+        buf = np.zeros(max(1, max_count) * self.n, dtype=np.intc)
+        k = C.c_int()
+        check(load().pbb_get_deep_subproblems(self._h, _ip(buf), max_count, C.byref(k)))
+        return buf[: k.value * self.n].reshape(k.value, self.n).copy()
+
     def save_checkpoint(self, path: str):
         """works::save: active intervals, incumbent, root order, counters."""
         check(load().pbb_checkpoint_save(self._h, path.encode()))

@@ -274,6 +274,28 @@ def test_checkpoint_save_load_resume(oracle_mod, tmp_path):
         eng2.close()
 
 
+def test_deep_subproblems(oracle_mod):
+    """getDeepSubproblem after a budgeted run: complete permutations of the
+    active explorers' current schedules, sorted by makespan; each schedule's
+    prefix/suffix is the explorer's exported interval position."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    ptm = O.synth_ptm(2, 50, 20)
+    neh, _ = O.Instance(ptm).neh()
+    eng = P.GpuBB(ptm, _cfg(P, 0, 1, 0, 0, nb=1024), ub=neh)
+    eng.init_full_interval()
+    assert not eng.next(node_budget=2_000_000, time_budget=60.0)
+    ids, pos, end, _, _ = eng.get_intervals()
+    sched = eng.get_deep_subproblems(64)
+    eng.close()
+    assert 0 < len(sched) <= min(64, len(ids))
+    costs = [P.makespan(ptm, s) for s in sched]
+    assert costs == sorted(costs)
+    for s in sched:
+        assert sorted(s.tolist()) == list(range(50))
+    assert len(P.GpuBB(ptm, _cfg(P, 0, 1, 0, 0, nb=64), ub=neh).get_deep_subproblems(8)) == 0
+
+
 def test_interrupt_and_continue(oracle_mod):
     import time
     import pbbperm_amd as P
