@@ -8,7 +8,11 @@ MaxSum branching, incumbent from NEH.  A step is one resident-engine launch
 large to finish, so the engine keeps its IVMs, intervals and incumbent in
 HBM across steps and each step continues the exploration.
 
-Two more legs, reported as objects of the same line:
+Three more legs, reported as objects of the same line:
+  "small_configs"  the 20x5 (LB1, NEH incumbent, 4096 explorers) and 20x10
+             (LB1 + Johnson LB2, incumbent = optimum, 16384 explorers)
+             parity configurations timed end to end, checked against the
+             reference's fixture, with the reference's single-thread time;
   "batched"  the 512-instance batch (seeds 100-611, n ~ U[10,22],
              m ~ U{5,10}, LB1 + Johnson LB2 over all machine pairs, MinMin
              branching, NEH incumbents) solved to optimality in one
@@ -128,6 +132,54 @@ def _params(PP, name, args):
     return best, src
 
 
+def leg_small(P, args):
+    """The two parity configurations timed end to end (one solve each):
+    20x5 seed 0, LB1, MaxSum, NEH incumbent, 4096 explorers; and 20x10
+    seed 1, LB1 + Johnson LB2, MaxSum, incumbent = the optimum, 16384
+    explorers.  Optimum and node counts against the reference's fixture
+    (counts are order-free only with the incumbent frozen at the optimum)."""
+    fx = os.path.join(ROOT, "tests", "golden", "solve_cases.json")
+    cases = json.load(open(fx)) if os.path.exists(fx) else []
+
+    def ref(seed, m, bm, br, rf, kind):
+        for c in cases:
+            if (c["seed"], c["m"], c["bound_mode"], c["branching"], c["refine"], c["ub_kind"]) == \
+                    (seed, m, bm, br, rf, kind):
+                return c
+        return None
+
+    out = {}
+    for name, seed, n, m, bm, rf, nb, kind in [("20x5_lb1_neh", 0, 20, 5, 0, 0, 4096, "neh"),
+                                               ("20x10_lb1_lb2_opt", 1, 20, 10, 2, 1, 16384, "opt")]:
+        ptm = P.synth(seed, n, m)
+        r = ref(seed, m, bm, 1, rf, kind)
+        ub = r["ub"] if r else P.neh(ptm)[0]
+        cfg = P.default_config(bound_mode=bm, branching=1, refine=rf, johnson_pairs=0, early_stop_johnson=0,
+                               nb_ivms=nb)
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            with P.GpuBB(ptm, cfg, ub=ub) as eng:
+                eng.solve(time_budget=60.0)
+                st = eng.stats()
+                best, _ = eng.best()
+            times.append(time.perf_counter() - t)
+        d = {"solve_ms": min(times) * 1e3, "explorers": st.explorers, "best": best, "decomposed": st.decomposed}
+        if r:
+            d["optimum_equals_reference"] = best == r["best"]
+            if kind == "opt":
+                d["counts_equal_reference"] = [st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds] == \
+                    [r["decomposed"], r["leaves"], r["simple_bounds"], r["johnson_bounds"]]
+        if not args.no_cpu:
+            import oracle as O
+            O.lib()
+            if O.have_ref():
+                rr = O.ref_solve(ptm, ub, bound_mode=bm, branching=1, refine=rf, johnson_pairs=0)
+                d["cpu_reference_ms"] = rr.wall_s * 1e3
+        out[name] = d
+    return out
+
+
 def leg_batched(P, PP, args):
     import numpy as np
     cfg = PP.workload_config("batched")
@@ -224,7 +276,7 @@ def main():
     ap.add_argument("--explorers", type=int, default=1 << 20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="nodes,batched,serving")
+    ap.add_argument("--legs", default="nodes,small,batched,serving")
     ap.add_argument("--serve-requests", type=int, default=600)
     ap.add_argument("--autotune", action="store_true", help="re-run the pool autotuner instead of tuned.json")
     ap.add_argument("--serve-load", type=float, default=0.8,
@@ -305,6 +357,9 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ptm, ub, args.cpu_seconds)
     legs = set(args.legs.split(","))
+    if "small" in legs:
+        line["small_configs"] = leg_small(P, args)
+        note(f"small configs: {line['small_configs']}")
     note(f"nodes leg: {value / 1e6:.1f} M nodes/s")
     if legs & {"batched", "serving"}:
         import pbbperm_amd.pool as PP
