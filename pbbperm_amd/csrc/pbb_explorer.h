@@ -67,6 +67,18 @@ __device__ __forceinline__ bool w_cas(unsigned *p, unsigned expect, unsigned des
     }
     return rdlane(ok, 0) != 0;
 }
+// Relaxed claim: no agent-scope release/acquire, so a failed attempt costs
+// one atomic round trip and no L2 writeback or invalidate.
+__device__ __forceinline__ bool w_cas_rlx(unsigned *p, unsigned expect, unsigned desired)
+{
+    int ok = 0;
+    if (lane_id() == 0) {
+        unsigned e = expect;
+        ok = __hip_atomic_compare_exchange_strong(p, &e, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return rdlane(ok, 0) != 0;
+}
 __device__ __forceinline__ void w_min64(unsigned long long *p, unsigned long long v)
 {
     if (lane_id() == 0) __hip_atomic_fetch_min(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
@@ -107,8 +119,9 @@ __device__ __forceinline__ int chain_scan(int base, int p)
 
 // ------------------------------------------------------------ LDS layout
 struct Lds {
-    int *pm;        // [m][ps] machine-major processing times
-    int ps;         // row stride of pm (odd)
+    int *pm;        // [m][PM_STRIDE] machine-major processing times
+    int *pmR;       // the same rows, machines reversed: pmR[q] = pm[m-1-q]
+    int ps;         // row stride of pm (PM_STRIDE)
     int *minA, *minD, *tot;
     int4 *jtab;     // Johnson table (LDS or global)
     int8_t *mat;    // this wave's IVM matrix, row l at mat + lds_mat_row(l, n)
@@ -121,12 +134,13 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
 {
     Lds L;
     const int n = I.n, mp = I.mp, m = I.m;
-    L.ps = lds_pm_stride(n);
+    L.ps = PM_STRIDE;
     L.pm = (int *)smem;
-    L.minA = L.pm + m * L.ps;
+    L.pmR = L.pm + m * PM_STRIDE;
+    L.minA = L.pmR + m * PM_STRIDE;
     L.minD = L.minA + mp;
     L.tot = L.minD + mp;
-    size_t off = ((size_t)m * L.ps + 3 * (size_t)mp) * 4;
+    size_t off = ((size_t)2 * m * PM_STRIDE + 3 * (size_t)mp) * 4;
     off = (off + 15) & ~(size_t)15;
     load_jtab = load_jtab && jtab_in_lds(n, I.npairs);
     L.jtab = load_jtab ? (int4 *)(smem + off) : const_cast<int4 *>(I.jtab);
@@ -138,7 +152,8 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
     L.scr = scratch ? (int *)(wbase + lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16) : nullptr;
     for (int i = threadIdx.x; i < n * m; i += blockDim.x) {
         const int q = i / n, j = i - q * n;
-        L.pm[q * L.ps + j] = I.ptm[i];
+        L.pm[q * PM_STRIDE + j] = I.ptm[i];
+        L.pmR[(m - 1 - q) * PM_STRIDE + j] = I.ptm[i];
     }
     for (int i = threadIdx.x; i < mp; i += blockDim.x) {
         L.minA[i] = i < m ? I.min_arr[i] : 0;
@@ -254,6 +269,7 @@ struct Explorer {
     int nf, nb;
     int local_best;
     unsigned dec, leaves, sb, jb;     // deltas since the last flush
+    unsigned nchk;                    // periodic checks so far (host-word phase)
     int hist;
     unsigned long long t_end;
 
@@ -265,6 +281,7 @@ struct Explorer {
         rowv = posv = endv = dirv = remv = 0;
         nf = nb = 0;
         dec = leaves = sb = jb = 0;
+        nchk = (unsigned)eid;  // staggers the host-word reads across explorers
         hist = 0;
         local_best = PBB_INF;
         t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
@@ -284,7 +301,7 @@ struct Explorer {
 
     __device__ __forceinline__ int pmach(int job) const  // lane q: p[q][job]
     {
-        return lane < M ? lds.pm[lane * lds.ps + job] : 0;
+        return lane < M ? lds.pm[lane * PM_STRIDE + job] : 0;
     }
 
     __device__ __forceinline__ void reset_stacks()
@@ -311,7 +328,7 @@ struct Explorer {
         } else {
             // back chain runs from machine M-1 down: lane q handles machine M-1-q
             int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
-            int pr = lane < M ? lds.pm[(M - 1 - lane) * lds.ps + j] : 0;
+            int pr = lane < M ? lds.pmR[lane * PM_STRIDE + j] : 0;
             int c = chain_scan(base, pr);
             if (lane < M) brow(nb + 1)[M - 1 - lane] = c;
             nb++;
@@ -372,12 +389,11 @@ struct Explorer {
         const int job = base == 0 ? bcast_low_half(rowv) : bcast_high_half(rowv);
         const int jb = c < k ? job : 0;
         // machine-major rows: front lanes walk machines 0..M-1, back lanes M-1..0
-        const int *pcol = lds.pm + (half ? (M - 1) * lds.ps : 0) + jb;
-        const int pstep = half ? -lds.ps : lds.ps;
+        const int *pcol = (half ? lds.pmR : lds.pm) + jb;
         const int2 *u = lds.U + half * M;
         int pj[M];
 #pragma unroll
-        for (int q = 0; q < M; q++) pj[q] = pcol[q * pstep];
+        for (int q = 0; q < M; q++) pj[q] = pcol[q * PM_STRIDE];
         int2 o = u[0];
         int t0 = o.x + pj[0];
         int lb = o.x + o.y;
@@ -509,8 +525,8 @@ struct Explorer {
     // d = 0: front' = front + job; d = 1: back' = job + back.
     __device__ __forceinline__ void child_chains(int job, int (&chF)[M], int (&chB)[M]) const
     {
-        const int *pj = lds.pm + job;  // machine q at pj[q * ps]
-        const int ps = lds.ps;
+        const int *pj = lds.pm + job;  // machine q at pj[q * PM_STRIDE]
+        constexpr int ps = PM_STRIDE;
         int t = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
@@ -530,8 +546,8 @@ struct Explorer {
     // child's front or back)
     __device__ __forceinline__ int lb1p_child(int d, int job, const int (&chF)[M], const int (&chB)[M]) const
     {
-        const int *pj = lds.pm + job;  // machine q at pj[q * ps]
-        const int ps = lds.ps;
+        const int *pj = lds.pm + job;  // machine q at pj[q * PM_STRIDE]
+        constexpr int ps = PM_STRIDE;
         int t0 = 0, lb = 0;
 #pragma unroll
         for (int q = 0; q < M; q++) {
@@ -569,7 +585,7 @@ struct Explorer {
         } else {
             // back chain over machines M-1..0: lane q handles machine M-1-q
             const int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
-            const int pr = lane < M ? lds.pm[(M - 1 - lane) * lds.ps + cj] : 0;
+            const int pr = lane < M ? lds.pmR[lane * PM_STRIDE + cj] : 0;
             const int cb = chain_scan(base, pr);
             if (lane < M) {
                 fs[lane] = frow(nf)[lane];
@@ -900,7 +916,13 @@ struct Explorer {
             else if (lane >= l && lane < n) endE = n - 1 - lane;
         }
         int nexp = __popcll(expl);
-        if (!w_cas(&ir.ctl->idle_head, h, h + 1)) return true;
+        // Every busy explorer that saw the ring non-empty races for ticket h.
+        // The claim orders nothing (the donor has published nothing yet, and
+        // the acquire load of the cell's sequence below orders the read of
+        // the thief's id), so it is relaxed: with thousands of contenders an
+        // acq_rel attempt, i.e. an L2 writeback + invalidate per loser,
+        // dominated the ramp-up of every solve.
+        if (!w_cas_rlx(&ir.ctl->idle_head, h, h + 1)) return true;
         // claim the idle explorer registered at ticket h (bounded MPMC ring cell)
         const unsigned slot = h & (unsigned)(ir.ring_cap - 1);
         bool ok = false;
@@ -1003,7 +1025,8 @@ struct Explorer {
         }
         if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
         if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(&run->stop, 1);
-        if (a.host_flag && lane == 0) {
+        const bool host_turn = (nchk++ & (PBB_HOST_POLL_CHECKS - 1u)) == 0u;
+        if (a.host_flag && host_turn && lane == 0) {
             int hf = __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             if (hf) st_rlx(&run->stop, 1);
         }

@@ -23,6 +23,17 @@ enum { PBB_DEV_ERR_RING = 1 };
 #define PBB_POLL_EVERY 1
 #endif
 
+// The host's interrupt word lives in host memory, so reading it is a PCIe
+// round trip.  A busy explorer reads it at every PBB_HOST_POLL_CHECKS-th
+// periodic check (one check per 64 decomposed nodes; explorers start at
+// staggered phases), and the first one to see it raises the device-side stop
+// word that every explorer reads at each check.  Measured at 50x20 with 4096
+// explorers: a host read every 64 nodes held the explorer at 720-1100 M
+// nodes/s, varying with the host's load; one every 1024 nodes, 1530-1545 M.
+#ifndef PBB_HOST_POLL_CHECKS
+#define PBB_HOST_POLL_CHECKS 16
+#endif
+
 // Device-resident control block of one engine (or pool slot).  The ring
 // tickets and the work counter take atomics from every donor; the flags are
 // polled by idle explorers; each group sits on its own 128-byte line so the
@@ -93,18 +104,20 @@ struct KArgs {
 };
 
 // LDS footprint.  Workgroup tables: the machine-major processing times
-// pm[m][ps] with an odd row stride ps = n | 1 (lanes reading one job on
-// different machines, or different jobs on one machine, hit distinct banks),
-// min_arr, min_dep, per-machine totals, and the Johnson table npairs*n int4
-// when LB2 is used and it fits.  Per wavefront: the IVM matrix (row l holds
-// its n-l cells, rows packed), the front/back stack ((n+1) x m ints), the
-// LB1 operand block (2 x m int2) and, for the LB2 variant, 64 ints of
-// scratch.
-__device__ __host__ inline int lds_pm_stride(int n) { return n | 1; }
+// pm[m][PM_STRIDE] and the same rows in reversed machine order pmR, with a
+// fixed odd row stride (lanes reading one job on different machines, or
+// different jobs on one machine, hit distinct banks, and every per-machine
+// offset is a compile-time immediate), min_arr, min_dep, per-machine totals,
+// and the Johnson table npairs*n int4 when LB2 is used and it fits.  Per
+// wavefront: the IVM matrix (row l holds its n-l cells, rows packed), the
+// front/back stack ((n+1) x m ints), the LB1 operand block (2 x m int2) and,
+// for the LB2 variant, 64 ints of scratch.
+#define PM_STRIDE 65
+__device__ __host__ inline int lds_pm_stride(int) { return PM_STRIDE; }
 __device__ __host__ inline size_t lds_table_bytes(int n, int m, int npairs, bool lb2)
 {
     const int mp = (m + 3) & ~3;
-    size_t off = ((size_t)m * lds_pm_stride(n) + 3 * (size_t)mp) * 4;
+    size_t off = ((size_t)2 * m * PM_STRIDE + 3 * (size_t)mp) * 4;
     off = (off + 15) & ~(size_t)15;
     if (lb2) off += (size_t)npairs * n * 16;
     return off;
@@ -129,6 +142,14 @@ inline size_t explore_lds_bytes(const DevInst &I, bool lb2)
 {
     return lds_table_bytes(I.n, I.m, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) + 4 * lds_wave_bytes(I.n, I.m, lb2);
 }
+// LDS size handed to the occupancy query of the persistent kernels.  Their
+// explorers wait on one another (the idle ring, work_count), so every
+// workgroup of the grid must be resident at once.  The query's count can be
+// one workgroup per CU too high when the workgroups fill the CU's LDS to
+// within a granule (12x20 with the Johnson table: 3 x 53904 B of 163840 B
+// queried as 3, held 2); rounding each workgroup up to 2 KB keeps the grid
+// inside what the hardware actually co-schedules.
+inline size_t residency_lds_bytes(size_t lds) { return (lds + 2047) & ~(size_t)2047; }
 inline size_t root_lds_bytes(const DevInst &I)
 {
     return lds_table_bytes(I.n, I.m, I.npairs, false) + lds_wave_bytes(I.n, I.m, false);

@@ -28,13 +28,15 @@
 namespace pbb {
 
 // ------------------------------------------------------------ kernels
-// Minimum resident waves per SIMD the compiler must allow (register budget).
+// Minimum resident waves per SIMD the compiler must allow (register
+// budget): 4 for the LB1 explorer (128 VGPRs), free for the refinement
+// variants.
 #ifndef PBB_EXPLORE_MIN_WAVES
-#define PBB_EXPLORE_MIN_WAVES 1
+#define PBB_EXPLORE_MIN_WAVES 4
 #endif
 
 template <int M, int REF>
-__global__ __launch_bounds__(256, PBB_EXPLORE_MIN_WAVES) void k_explore(KArgs a)
+__global__ __launch_bounds__(256, REF == 0 ? PBB_EXPLORE_MIN_WAVES : 1) void k_explore(KArgs a)
 {
     extern __shared__ __align__(16) char smem[];
     Lds lds = lds_setup(a.I, smem, REF == 2, REF == 2);
@@ -62,8 +64,11 @@ __global__ __launch_bounds__(256, PBB_EXPLORE_MIN_WAVES) void k_explore(KArgs a)
             // wait for a donation, global termination or a stop
             int nap = 1;
             while (true) {
-                int s = uni(ld_acq(a.state + e));
+                // relaxed poll (an acquire would invalidate the L2 at every
+                // pass); the fence pairs with the donor's release of the word
+                int s = uni(ld_rlx(a.state + e));
                 if (s == ST_ACTIVE) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     st = ST_ACTIVE;
                     X.load_state();
                     break;
@@ -278,12 +283,13 @@ hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
         *blocks_per_cu = 0;
         return hipSuccess;
     }
-    hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const size_t q = residency_lds_bytes(lds);
+    hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)q);
     if (a.bound_mode == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 0>, 256, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 0>, 256, q);
     if (a.refine == 1)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 2>, 256, lds);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 1>, 256, lds);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 2>, 256, q);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 1>, 256, q);
 }
 
 template <int M>

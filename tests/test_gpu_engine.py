@@ -114,6 +114,30 @@ def test_explorer_count_invariance(oracle_mod):
         assert best == opt and _same_counts(st, ref)
 
 
+def test_full_grid_at_the_lds_edge():
+    """12x20 with LB2 and the Johnson table in LDS fills a CU's LDS to within
+    2 KB at three workgroups.  The persistent grid must still be co-resident:
+    with one workgroup per CU too many, the resident explorers idle while the
+    rest hold work, and the solve runs into its deadline.  Counts are the
+    committed frozen-bound reference counts of that request."""
+    import json
+    import os
+    import pbbperm_amd as P
+    with open(os.path.join(os.path.dirname(__file__), "golden", "serve_small.json")) as f:
+        fx = json.load(f)
+    sp = fx["spec"]
+    d = next(x for x in fx["instances"] if x["seed"] == 20194)
+    ptm = P.synth(d["seed"], d["n"], d["m"])
+    for nb in (4096, 100000):
+        eng = P.GpuBB(ptm, _cfg(P, sp["bound_mode"], sp["branching"], sp["refine"], sp["pairs"], nb), ub=d["opt"])
+        done = eng.solve(time_budget=20.0)
+        st = eng.stats()
+        eng.close()
+        print(f"nb_ivms={nb} explorers={st.explorers} done={done} dec={st.decomposed}", flush=True)
+        assert done
+        assert [st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds] == d["counts_frozen"]
+
+
 def test_config1_20x5_lb1_neh(oracle_mod):
     """20 jobs x 5 machines, seed 0, LB1, UB from NEH, 4096 explorers."""
     import pbbperm_amd as P
