@@ -17,10 +17,11 @@ enum { PBB_DEV_ERR_RING = 1 };
 
 // A busy explorer reads the idle ring every PBB_POLL_EVERY decomposed nodes
 // while it was empty at the last read, and at every node while explorers
-// wait in it (power of two).  Measured at 50x20: every node is fastest
-// (747-752 M nodes/s against 687-709 M every 8th and 676-700 M every 32nd).
+// wait in it (power of two).  Measured: the 50x20 explorer is the same at 1,
+// 4 and 16 (1500-1530 M nodes/s); the 512-instance pool batch takes 500-518
+// ms at 1, 459-490 at 4 and 433-438 at 16.
 #ifndef PBB_POLL_EVERY
-#define PBB_POLL_EVERY 1
+#define PBB_POLL_EVERY 16
 #endif
 
 // The host's interrupt word lives in host memory, so reading it is a PCIe

@@ -203,18 +203,31 @@ GRID_EPI = (256, 1024, 64)
 GRID_MARGIN = (2, 4, 8)
 
 
+OBJECTIVES = ("makespan", "replay_mean")
+
+
 def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Config] = None,
-             base: Optional[PoolParams] = None, time_budget: float = 120.0, progress=None):
+             base: Optional[PoolParams] = None, time_budget: float = 120.0, progress=None,
+             objective: str = "makespan", arrivals: Optional[Sequence[float]] = None):
     """Pick waves per workgroup (block size), explorers per instance and the
-    steal margin that solve the given instances fastest: every candidate of
-    GRID_W x GRID_EPI x GRID_MARGIN is timed as the best of two graph
-    launches after a warm one, each capped at twice the best time so far plus
-    0.5 s (a capped launch fails).
+    steal margin for the given instances; every candidate of GRID_W x
+    GRID_EPI x GRID_MARGIN is scored, each run capped at twice the best run so
+    far plus 0.5 s (a capped run fails).
+    objective "makespan": the instances solved as one batch, best of two
+    graph launches after a warm one.  "replay_mean": the instances replayed
+    as a request stream arriving at `arrivals` (s) through the resident
+    serving loop, scored by the mean request latency (one replay per
+    candidate, after one warm replay per block size).
     Returns (params, trials) with trials rows (W, explorers_per_instance,
-    margin, ms).  progress(row) is called after each trial."""
+    margin, score ms).  progress(row) is called after each trial."""
+    if objective not in OBJECTIVES:
+        raise ValueError(f"objective must be one of {OBJECTIVES}")
+    if objective == "replay_mean" and (arrivals is None or len(arrivals) != len(ptms)):
+        raise ValueError("replay_mean needs one arrival time per instance")
     cfg = config if config is not None else default_config()
     b = base if base is not None else default_params()
     trials = []
+    best_run = float("inf")
     for W in GRID_W:
         prm = PoolParams()
         C.memmove(C.byref(prm), C.byref(b), C.sizeof(PoolParams))
@@ -223,20 +236,29 @@ def autotune(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Optional[Co
         prm.slots = max(b.slots, len(ptms))
         with Pool(cfg, params=prm) as pool:
             E = pool.info()["explorers"]
+            if objective == "replay_mean":
+                k = min(16, len(ptms))
+                pool.replay(ptms[:k], ubs[:k], np.zeros(k), time_budget=time_budget)
             for epi in GRID_EPI:
                 if epi < W or epi > E:
                     continue
                 for mg in GRID_MARGIN:
                     pool.retune(epi, mg)
-                    # a trial may not run longer than twice the best so far (+0.5 s)
-                    done = [t[3] for t in trials if np.isfinite(t[3])]
-                    cap = min(time_budget, 2.0 * min(done) / 1e3 + 0.5) if done else time_budget
-                    pool.solve(ptms, ubs, time_budget=cap)
+                    # a run may not last longer than twice the best so far (+0.5 s)
+                    cap = min(time_budget, 2.0 * best_run / 1e3 + 0.5) if np.isfinite(best_run) else time_budget
                     ms = float("inf")
-                    for _ in range(2):  # best of two timed launches (the search order varies)
-                        res, t = pool.solve(ptms, ubs, time_budget=cap)
+                    if objective == "makespan":
+                        pool.solve(ptms, ubs, time_budget=cap)
+                        for _ in range(2):  # best of two timed launches (the search order varies)
+                            res, t = pool.solve(ptms, ubs, time_budget=cap)
+                            if all(r.status == "solved" for r in res):
+                                best_run = min(best_run, t)
+                                ms = min(ms, t)
+                    else:
+                        res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=cap)
                         if all(r.status == "solved" for r in res):
-                            ms = min(ms, t)
+                            best_run = min(best_run, wall * 1e3)
+                            ms = float(np.mean(lat))
                     row = (W, epi, mg, ms)
                     trials.append(row)
                     if progress:
@@ -318,9 +340,9 @@ _LB12 = dict(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_jo
 
 WORKLOADS = {
     "batched": dict(config=_LB12, max_jobs=22, machines=10, slots=512,
-                    tune=lambda: batch_stream(512, 1100)),
+                    tune=lambda: batch_stream(512, 1100), objective="makespan"),
     "serving": dict(config=_LB12, max_jobs=30, machines=20, slots=256,
-                    tune=lambda: serving_stream(64, 30000)),
+                    tune=lambda: serving_stream(120, 30000), objective="replay_mean", load=0.8),
 }
 
 TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
@@ -343,9 +365,17 @@ def tune_workload(name: str, progress=None):
     """Run the autotuner on the workload's tuning sample (disjoint seeds from
     the measured stream); returns (params, trials)."""
     from .engine import neh
-    sample = WORKLOADS[name]["tune"]()
+    w = WORKLOADS[name]
+    sample = w["tune"]()
     ubs = [neh(p)[0] for p in sample]
-    return autotune(sample, ubs, config=workload_config(name), base=workload_params(name), progress=progress)
+    arrivals = None
+    if w["objective"] == "replay_mean":
+        # arrivals at `load` times the plain path's closed-loop rate on the sample
+        _, _, wall = plain_replay(sample, ubs, np.zeros(len(sample)), config=workload_config(name, nb_ivms=1024),
+                                  time_budget=300.0)
+        arrivals = poisson_arrivals(len(sample), w["load"] * len(sample) / wall, seed=11)
+    return autotune(sample, ubs, config=workload_config(name), base=workload_params(name), progress=progress,
+                    objective=w["objective"], arrivals=arrivals)
 
 
 def tuned_params(name: str, path: str = TUNED_PATH) -> Optional[PoolParams]:

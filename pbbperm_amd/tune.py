@@ -2,11 +2,14 @@
 
   python -m pbbperm_amd.tune [--out PATH] [batched] [serving]
 
-For each workload the autotuner (pbbperm_amd.pool.autotune) times every
+For each workload the autotuner (pbbperm_amd.pool.autotune) scores every
 (waves per workgroup, explorers per instance, steal margin) candidate on a
 tuning sample of the workload's distribution with seeds disjoint from the
-measured stream, and the fastest is written to PATH (default
-pbbperm_amd/tuned.json) with the trial table.  bench.py reads that file.
+measured stream -- the batch's makespan for "batched"; for "serving", the
+mean latency of a Poisson replay of the sample at 0.8 x the plain path's
+closed-loop rate -- and the best is written to PATH
+(default pbbperm_amd/tuned.json) with the trial table.  bench.py reads that
+file.
 """
 from __future__ import annotations
 
@@ -38,6 +41,7 @@ def main(argv=None):
             "explorers_per_instance": best.explorers_per_instance,
             "steal_margin": best.steal_margin,
             "trials": [[int(r[0]), int(r[1]), int(r[2]), (float(r[3]) if r[3] < 1e300 else None)] for r in trials],
+            "objective": PP.WORKLOADS[name]["objective"],
             "tune_s": round(time.time() - t0, 1),
         }
         print(json.dumps({name: data[name]}), flush=True)

@@ -287,6 +287,19 @@ def test_autotuner_picks_a_valid_configuration(oracle_mod):
     with PP.Pool(_cfg(P, 2, 3, 1, 0), params=best) as pool:
         res, _ = pool.solve(ptms, opts, time_budget=BUDGET)
     assert all(r.status == "solved" and r.best == u for r, u in zip(res, opts))
+    # the latency objective replays the instances as a request stream and
+    # searches the same grid
+    arrivals = PP.poisson_arrivals(len(ptms), 400.0, seed=3)
+    rbest, rtrials = PP.autotune(ptms, opts, config=_cfg(P, 2, 3, 1, 0), base=base, time_budget=BUDGET,
+                                 objective="replay_mean", arrivals=arrivals)
+    assert [tuple(r[:3]) for r in rtrials] == [tuple(r[:3]) for r in trials]
+    assert np.all(np.isfinite(rtrials[:, 3])) and np.all(rtrials[:, 3] > 0)
+    assert (rbest.waves_per_group, rbest.explorers_per_instance, rbest.steal_margin) in \
+        [tuple(int(x) for x in r[:3]) for r in rtrials]
+    with pytest.raises(ValueError):
+        PP.autotune(ptms, opts, objective="p99")
+    with pytest.raises(ValueError):
+        PP.autotune(ptms, opts, objective="replay_mean")
 
 
 def test_device_neh_matches_host_neh():
