@@ -270,6 +270,7 @@ struct Explorer {
     int local_best;
     unsigned dec, leaves, sb, jb;     // deltas since the last flush
     unsigned nchk;                    // periodic checks so far (host-word phase)
+    bool end_dirty;                   // end moved since pos was last compared with it
     int hist;
     unsigned long long t_end;
 
@@ -282,6 +283,7 @@ struct Explorer {
         nf = nb = 0;
         dec = leaves = sb = jb = 0;
         nchk = (unsigned)eid;  // staggers the host-word reads across explorers
+        end_dirty = true;
         hist = 0;
         local_best = PBB_INF;
         t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
@@ -758,27 +760,40 @@ struct Explorer {
         }
     }
 
-    // One call of sequentialbb::next: returns 1 when an interior node was
-    // decomposed and bounded, 0 when the interval is exhausted.
+    // One call of sequentialbb::next (sequentialbb.cpp:128-163): returns 1
+    // when an interior node was decomposed and bounded, 0 when the interval
+    // is exhausted.  The reference tests beforeEnd at every iteration; pos
+    // only grows in goRight and at the root line's increment (goUp resets a
+    // deeper position to 0, goDown opens a line at 0), and end only shrinks
+    // when a donation trims it, so the test runs exactly there and gives the
+    // same answers.  goRight then goes down or up in the same iteration.
     __device__ __forceinline__ int step()
     {
-        while (true) {
+        if (end_dirty) {
+            end_dirty = false;
             if (!before_end()) return 0;
+        }
+        while (true) {
             const int p = rdlane(posv, L);
-            if (p >= n - L) {
-                go_up();
-                continue;
+            int c = p;
+            if (p < n - L) {
+                // first live cell at or after p (ivm::goRight over pruned / visited cells)
+                const uint64_t live = ballot(lane < n - L && rowv >= 0);
+                const uint64_t ahead = live & (~0ull << p);
+                c = ahead ? __ffsll((unsigned long long)ahead) - 1 : n - L;
+                if (c != p) {
+                    posv = wrlane(posv, c, L);
+                    if (!before_end()) return 0;
+                }
             }
-            uint64_t live = ballot(lane < n - L && rowv >= 0);
-            uint64_t ahead = live & (~0ull << p);
-            if (!((ahead >> p) & 1ull)) {
-                // goRight across pruned / visited cells (ivm::goRight)
-                int c = ahead ? __ffsll((unsigned long long)ahead) - 1 : n - L;
-                posv = wrlane(posv, c, L);
+            if (c >= n - L) {
+                const bool root_line = L == 0;
+                go_up();
+                if (root_line && !before_end()) return 0;
                 continue;
             }
             dec++;
-            go_down(p, true);
+            go_down(c, true);
             hist += (lane == L);
             if (L == n - 1) {
                 leaf();
@@ -810,6 +825,7 @@ struct Explorer {
         posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
         endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
         dirv = lane < n ? (int)a.dir[(size_t)e * 64 + lane] : 0;
+        end_dirty = true;
         __builtin_amdgcn_wave_barrier();
         rebuild_stacks();
     }
@@ -837,6 +853,7 @@ struct Explorer {
         posv = lane < n ? (int)a.pos[(size_t)e * 64 + lane] : 0;
         endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
         dirv = lane == 0 ? ir.root_dir[0] : 0;
+        end_dirty = true;
         reset_stacks();
         if (!before_end()) return false;
         unfold();
@@ -852,6 +869,7 @@ struct Explorer {
         posv = 0;
         endv = lane < n ? n - 1 - lane : 0;
         dirv = lane == 0 ? dirv : 0;
+        end_dirty = true;
         reset_stacks();
         if (!before_end()) return false;
         unfold();
@@ -966,6 +984,7 @@ struct Explorer {
         endv = endE;
         if (lane == l) endv = cut - 1;
         if (lane > l && lane < n) endv = n - 1 - lane;
+        end_dirty = true;
         w_add(&ir.ctl->work_count, 1);
         if (lane == 0) st_rel(a.state + thief, ST_ACTIVE);
         return true;
