@@ -267,6 +267,7 @@ struct Explorer {
     int L;                            // current line
     int rowv, posv, endv, dirv, remv; // lane vectors (remv: lane q = remaining work on machine q)
     int nf, nb;
+    int ftop;                         // lane q < M: top front entry f_q (frow(nf)), else 0
     int local_best;
     unsigned dec, leaves, sb, jb;     // deltas since the last flush
     unsigned nchk;                    // periodic checks so far (host-word phase)
@@ -281,6 +282,7 @@ struct Explorer {
         L = 0;
         rowv = posv = endv = dirv = remv = 0;
         nf = nb = 0;
+        ftop = 0;
         dec = leaves = sb = jb = 0;
         nchk = (unsigned)eid;  // staggers the host-word reads across explorers
         end_dirty = true;
@@ -309,8 +311,9 @@ struct Explorer {
     __device__ __forceinline__ void reset_stacks()
     {
         nf = nb = 0;
+        ftop = lane < M ? lds.minA[lane] : 0;
         if (lane < M) {
-            frow(0)[lane] = lds.minA[lane];
+            frow(0)[lane] = ftop;
             brow(0)[lane] = lds.minD[lane];
         }
         remv = lane < M ? lds.tot[lane] : 0;
@@ -323,9 +326,9 @@ struct Explorer {
     {
         const int pk = pmach(j);
         if (d == 0) {
-            int base = lane < M ? frow(nf)[lane] : 0;
-            int c = chain_scan(base, pk);
+            int c = chain_scan(ftop, pk);
             if (lane < M) frow(nf + 1)[lane] = c;
+            ftop = lane < M ? c : 0;
             nf++;
         } else {
             // back chain runs from machine M-1 down: lane q handles machine M-1-q
@@ -341,10 +344,12 @@ struct Explorer {
 
     __device__ __forceinline__ void pop_job(int j, int d)
     {
-        if (d == 0)
+        if (d == 0) {
             nf--;
-        else
+            ftop = lane < M ? frow(nf)[lane] : 0;
+        } else {
             nb--;
+        }
         remv += pmach(j);
     }
 
@@ -373,7 +378,7 @@ struct Explorer {
     __device__ __forceinline__ void build_operands() const
     {
         if (lane < M) {
-            const int f = frow(nf)[lane], b = brow(nb)[lane], r = remv;
+            const int f = ftop, b = brow(nb)[lane], r = remv;
             lds.U[lane] = make_int2(f, r + b);
             lds.U[M + (M - 1 - lane)] = make_int2(b, f + r);
         }
@@ -578,7 +583,7 @@ struct Explorer {
         const int cj = flag_abs(rdlane(rowv, c));
         int *fs = lds.scr, *bs = lds.scr + 32;
         if (d == 0) {
-            const int f = lane < M ? frow(nf)[lane] : 0;
+            const int f = ftop;
             const int cf = chain_scan(f, pmach(cj));
             if (lane < M) {
                 fs[lane] = cf;
@@ -590,7 +595,7 @@ struct Explorer {
             const int pr = lane < M ? lds.pmR[lane * PM_STRIDE + cj] : 0;
             const int cb = chain_scan(base, pr);
             if (lane < M) {
-                fs[lane] = frow(nf)[lane];
+                fs[lane] = ftop;
                 bs[M - 1 - lane] = cb;
             }
         }
@@ -672,8 +677,7 @@ struct Explorer {
     // last row job + back; the incumbent only moves on strict improvement.
     __device__ __forceinline__ int makespan_with(int r) const
     {
-        int base = lane < M ? frow(nf)[lane] : 0;
-        int c = chain_scan(base, pmach(r));
+        int c = chain_scan(ftop, pmach(r));
         int b = lane < M ? brow(nb)[lane] : 0;
         return wave_max(lane < M ? c + b : 0);
     }

@@ -596,7 +596,7 @@ static int capacity(pbb_engine *e, int *explorers)
     HIPCHK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
-    *explorers = bpc * prop.multiProcessorCount * 4;
+    *explorers = bpc * prop.multiProcessorCount * PBB_EXPLORE_WAVES;
     return PBB_OK;
 }
 
@@ -662,7 +662,7 @@ extern "C" int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cf
     int cap = 0;
     TRY(capacity(e, &cap));
     e->E = std::max(1, std::min(cfg->nb_ivms, cap));
-    e->blocks = (e->E + 3) / 4;
+    e->blocks = (e->E + PBB_EXPLORE_WAVES - 1) / PBB_EXPLORE_WAVES;
     e->ring_cap = 1;
     while (e->ring_cap < 2 * e->E) e->ring_cap <<= 1;
     size_t E = (size_t)e->E;

@@ -139,9 +139,15 @@ __device__ __host__ inline size_t lds_wave_bytes(int n, int m, bool scratch)
 // The Johnson table is staged in LDS when it is at most 64 KB (m <= 10 at
 // n <= 50, m = 20 at n <= 21); larger tables are read from global memory.
 __device__ __host__ inline bool jtab_in_lds(int n, int npairs) { return (size_t)npairs * n * 16 <= 64 * 1024; }
+// Explorers (wavefronts) per workgroup of the one-instance engine; they share
+// the workgroup's tables.
+#ifndef PBB_EXPLORE_WAVES
+#define PBB_EXPLORE_WAVES 4
+#endif
 inline size_t explore_lds_bytes(const DevInst &I, bool lb2)
 {
-    return lds_table_bytes(I.n, I.m, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) + 4 * lds_wave_bytes(I.n, I.m, lb2);
+    return lds_table_bytes(I.n, I.m, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) +
+           PBB_EXPLORE_WAVES * lds_wave_bytes(I.n, I.m, lb2);
 }
 // LDS size handed to the occupancy query of the persistent kernels.  Their
 // explorers wait on one another (the idle ring, work_count), so every

@@ -36,11 +36,11 @@ namespace pbb {
 #endif
 
 template <int M, int REF>
-__global__ __launch_bounds__(256, REF == 0 ? PBB_EXPLORE_MIN_WAVES : 1) void k_explore(KArgs a)
+__global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_WAVES : 1) void k_explore(KArgs a)
 {
     extern __shared__ __align__(16) char smem[];
     Lds lds = lds_setup(a.I, smem, REF == 2, REF == 2);
-    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int e = blockIdx.x * PBB_EXPLORE_WAVES + (threadIdx.x >> 6);
     if (e >= a.E) return;
 
     Explorer<M, REF> X(a, lds, e, engine_inst(a), a.ctl, a.I.n, a.I.mp);
@@ -266,11 +266,11 @@ hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s)
     const size_t lds = explore_lds_bytes(a.I, lb2);
     hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (a.bound_mode == 0)
-        hipLaunchKernelGGL((k_explore<M, 0>), dim3(blocks), dim3(256), lds, s, a);
+        hipLaunchKernelGGL((k_explore<M, 0>), dim3(blocks), dim3(64 * PBB_EXPLORE_WAVES), lds, s, a);
     else if (a.refine == 1)
-        hipLaunchKernelGGL((k_explore<M, 2>), dim3(blocks), dim3(256), lds, s, a);
+        hipLaunchKernelGGL((k_explore<M, 2>), dim3(blocks), dim3(64 * PBB_EXPLORE_WAVES), lds, s, a);
     else
-        hipLaunchKernelGGL((k_explore<M, 1>), dim3(blocks), dim3(256), lds, s, a);
+        hipLaunchKernelGGL((k_explore<M, 1>), dim3(blocks), dim3(64 * PBB_EXPLORE_WAVES), lds, s, a);
     return hipGetLastError();
 }
 
@@ -286,10 +286,10 @@ hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
     const size_t q = residency_lds_bytes(lds);
     hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)q);
     if (a.bound_mode == 0)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 0>, 256, q);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 0>, 64 * PBB_EXPLORE_WAVES, q);
     if (a.refine == 1)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 2>, 256, q);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 1>, 256, q);
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 2>, 64 * PBB_EXPLORE_WAVES, q);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 1>, 64 * PBB_EXPLORE_WAVES, q);
 }
 
 template <int M>
