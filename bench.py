@@ -50,7 +50,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per decomposed node of k_explore on this workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
 # plain 256 MiB device copy): profiles/k_explore_50x20_lb1_pmc.md
-HBM_BYTES_PER_NODE = 0.82
+HBM_BYTES_PER_NODE = 0.80
 
 
 def lb1_ops(depth: int, n: int, m: int) -> int:
