@@ -150,36 +150,64 @@ __device__ __forceinline__ int lb1_node_vec(const int (&f)[M], const int (&r)[M]
     return lb;
 }
 
-// Johnson two-machine bound over all machine pairs, one pair per lane
-// (bound_fsp_strong.cpp:50-100, :102-162 without early exit):
-// front/back are uniform vectors staged in LDS (fs/bs, M ints each),
-// unsched is the bitmask of unscheduled job ids (n <= 64).
-// pairs_mode: JohnsonMode 0 all, 1 (i, m-1), 2 (i, i+1).
-__device__ __forceinline__ int lb2_pairs_wave(const DevInst &I, const int *fs, const int *bs,
-                                              uint64_t unsched, int pairs_mode)
+// Machine pairs of the Johnson bound, one per lane and chunk: chunk c of
+// lane l holds pair c*64 + l of the list as a | b << 8, or -1 when there is
+// no such pair or the JohnsonMode drops it (pairs_mode 1: pairs ending on the
+// last machine; 2: adjacent machines; bound_fsp_strong.cpp:282-297).  Built
+// once per explorer, so the pair list is not re-read for every child.
+template <int PCH>
+__device__ __forceinline__ void pair_lanes(const int *pair_ab, int npairs, int m, int pairs_mode, int (&pinfo)[PCH])
+{
+    const int lane = lane_id();
+#pragma unroll
+    for (int c = 0; c < PCH; c++) {
+        const int p = c * PBB_WAVE + lane;
+        int v = -1;
+        if (p < npairs) {
+            const int ab = pair_ab[p];
+            const int a = ab & 0xff, b = ab >> 8;
+            bool use = true;
+            if (pairs_mode == 1 && b < m - 1) use = false;
+            if (pairs_mode == 2 && b != a + 1) use = false;
+            if (use) v = a | (b << 8);
+        }
+        pinfo[c] = v;
+    }
+}
+
+// Johnson LB2 of one partial schedule over the machine pairs of pinfo
+// (bound_fsp_strong.cpp:50-100, :102-162 without early exit, as
+// bornes_calculer calls them, :540-560): fs / bs are its front and back completion
+// times (LDS, indexed by machine), unsched the bit set of its unscheduled
+// jobs, jtab the per-pair Johnson order {job, p_a, p_b, lag} of all n jobs.
+// Each lane runs its pair's two-machine recurrence over the unscheduled jobs
+// in that order; the loop body is predicated, so a lane never branches on a
+// job, and jtab is read with one 16-byte load per job (ds_read_b128 when the
+// caller passes the LDS copy: a pointer that may be either would make every
+// access a flat load).
+template <int PCH>
+__device__ __forceinline__ int lb2_pairs(const int (&pinfo)[PCH], const int4 *jtab, int n, const int *fs,
+                                         const int *bs, uint64_t unsched)
 {
     const int lane = lane_id();
     int best = 0;
-    for (int base = 0; base < I.npairs; base += PBB_WAVE) {
-        int p = base + lane;
-        if (p < I.npairs) {
-            int ab = I.pair_ab[p];
-            int a = ab & 0xff, b = ab >> 8;
-            bool use = true;
-            if (pairs_mode == 1 && b < I.m - 1) use = false;
-            if (pairs_mode == 2 && b != a + 1) use = false;
-            if (use) {
-                int t0 = fs[a], t1 = fs[b];
-                const int4 *row = I.jtab + (size_t)p * I.n;
-                for (int i = 0; i < I.n; i++) {
-                    int4 e = row[i];
-                    if ((unsched >> e.x) & 1ull) {
-                        t0 += e.y;
-                        t1 = max(t1, t0 + e.w) + e.z;
-                    }
-                }
-                best = max(best, max(t1 + bs[b], t0 + bs[a]));
+#pragma unroll
+    for (int c = 0; c < PCH; c++) {
+        const int info = pinfo[c];
+        if (info >= 0) {
+            const int a = info & 0xff, b = info >> 8;
+            int t0 = fs[a], t1 = fs[b];
+            const int4 *row = jtab + (size_t)(c * PBB_WAVE + lane) * n;
+#pragma unroll 4
+            for (int i = 0; i < n; i++) {
+                const int4 e = row[i];
+                const bool use = (unsched >> e.x) & 1ull;
+                const int u0 = t0 + e.y;
+                const int u1 = max(t1, u0 + e.w) + e.z;
+                t0 = use ? u0 : t0;
+                t1 = use ? u1 : t1;
             }
+            best = max(best, max(t1 + bs[b], t0 + bs[a]));
         }
     }
     return wave_max(best);

@@ -123,7 +123,9 @@ struct Lds {
     int *pmR;       // the same rows, machines reversed: pmR[q] = pm[m-1-q]
     int ps;         // row stride of pm (PM_STRIDE)
     int *minA, *minD, *tot;
-    int4 *jtab;     // Johnson table (LDS or global)
+    int4 *jtabL;    // Johnson table staged in LDS (valid when jtl)
+    const int4 *jtabG;  // the same table in global memory
+    bool jtl;
     int8_t *mat;    // this wave's IVM matrix, row l at mat + lds_mat_row(l, n)
     int *stk;       // this wave's stacks: F entry s at row s, B entry s at row n-s ((n+1) x m)
     int2 *U;        // this wave's LB1 operand block [2][m]
@@ -143,7 +145,9 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
     size_t off = ((size_t)2 * m * PM_STRIDE + 3 * (size_t)mp) * 4;
     off = (off + 15) & ~(size_t)15;
     load_jtab = load_jtab && jtab_in_lds(n, I.npairs);
-    L.jtab = load_jtab ? (int4 *)(smem + off) : const_cast<int4 *>(I.jtab);
+    L.jtabL = (int4 *)(smem + off);
+    L.jtabG = I.jtab;
+    L.jtl = load_jtab;
     char *wbase = smem + lds_table_bytes(n, m, I.npairs, load_jtab) +
                   (size_t)(threadIdx.x >> 6) * lds_wave_bytes(n, m, scratch);
     L.mat = (int8_t *)wbase;
@@ -164,7 +168,7 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
         L.tot[i] = s;
     }
     if (load_jtab)
-        for (int i = threadIdx.x; i < I.npairs * n; i += blockDim.x) L.jtab[i] = I.jtab[i];
+        for (int i = threadIdx.x; i < I.npairs * n; i += blockDim.x) L.jtabL[i] = I.jtab[i];
     __syncthreads();
     return L;
 }
@@ -268,6 +272,8 @@ struct Explorer {
     int rowv, posv, endv, dirv, remv; // lane vectors (remv: lane q = remaining work on machine q)
     int nf, nb;
     int ftop;                         // lane q < M: top front entry f_q (frow(nf)), else 0
+    static constexpr int PCH = (M * (M - 1) / 2 + PBB_WAVE - 1) / PBB_WAVE;  // pair chunks (M >= 2)
+    int pinfo[PCH];                   // this lane's Johnson machine pairs (REF 2)
     int local_best;
     unsigned dec, leaves, sb, jb;     // deltas since the last flush
     unsigned nchk;                    // periodic checks so far (host-word phase)
@@ -283,6 +289,7 @@ struct Explorer {
         rowv = posv = endv = dirv = remv = 0;
         nf = nb = 0;
         ftop = 0;
+        if constexpr (REF == 2) pair_lanes<PCH>(r.pair_ab, r.npairs, r.m, r.pairs_mode, pinfo);
         dec = leaves = sb = jb = 0;
         nchk = (unsigned)eid;  // staggers the host-word reads across explorers
         end_dirty = true;
@@ -610,13 +617,8 @@ struct Explorer {
             r = wave_max(lane < M ? cc + bq : 0);
         } else {
             const uint64_t unsched = rowmask & ~(1ull << cj);
-            DevInst J{};
-            J.n = n;
-            J.m = ir.m;
-            J.npairs = ir.npairs;
-            J.pair_ab = ir.pair_ab;
-            J.jtab = lds.jtab;
-            r = lb2_pairs_wave(J, fs, bs, unsched, ir.pairs_mode);
+            r = lds.jtl ? lb2_pairs<PCH>(pinfo, lds.jtabL, n, fs, bs, unsched)
+                        : lb2_pairs<PCH>(pinfo, lds.jtabG, n, fs, bs, unsched);
         }
         __builtin_amdgcn_wave_barrier();
         return r;

@@ -244,9 +244,10 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
             bs[lane] = X.brow(X.nb)[lane];
         }
         __builtin_amdgcn_wave_barrier();
-        DevInst J = a.I;
-        J.jtab = lds.jtab;
-        int v = lb2_pairs_wave(J, fs, bs, um, arg);
+        int pinfo[Explorer<M, 2>::PCH];
+        pair_lanes<Explorer<M, 2>::PCH>(a.I.pair_ab, a.I.npairs, a.I.m, arg, pinfo);
+        int v = lds.jtl ? lb2_pairs<Explorer<M, 2>::PCH>(pinfo, lds.jtabL, a.I.n, fs, bs, um)
+                        : lb2_pairs<Explorer<M, 2>::PCH>(pinfo, lds.jtabG, a.I.n, fs, bs, um);
         if (lane == 0) o1[s] = v;
     }
 }
