@@ -184,11 +184,13 @@ __device__ __forceinline__ void pair_lanes(const int *pair_ab, int npairs, int m
 // in that order; the loop body is predicated, so a lane never branches on a
 // job, and jtab is read with one 16-byte load per job (ds_read_b128 when the
 // caller passes the LDS copy: a pointer that may be either would make every
-// access a flat load).
-template <int PCH>
+// access a flat load).  N32: n <= 32, the membership test is one bit
+// extract of the low word.
+template <int PCH, bool N32>
 __device__ __forceinline__ int lb2_pairs(const int (&pinfo)[PCH], const int4 *jtab, int n, const int *fs,
                                          const int *bs, uint64_t unsched)
 {
+    const unsigned unsched32 = (unsigned)unsched;
     const int lane = lane_id();
     int best = 0;
 #pragma unroll
@@ -201,7 +203,11 @@ __device__ __forceinline__ int lb2_pairs(const int (&pinfo)[PCH], const int4 *jt
 #pragma unroll 4
             for (int i = 0; i < n; i++) {
                 const int4 e = row[i];
-                const bool use = (unsched >> e.x) & 1ull;
+                bool use;
+                if constexpr (N32)
+                    use = (unsched32 >> e.x) & 1u;
+                else
+                    use = (unsched >> e.x) & 1ull;
                 const int u0 = t0 + e.y;
                 const int u1 = max(t1, u0 + e.w) + e.z;
                 t0 = use ? u0 : t0;
@@ -211,6 +217,19 @@ __device__ __forceinline__ int lb2_pairs(const int (&pinfo)[PCH], const int4 *jt
         }
     }
     return wave_max(best);
+}
+
+// lb2_pairs with the table in LDS (jl) or in global memory, for any n
+template <int PCH>
+__device__ __forceinline__ int lb2_pairs_any(const int (&pinfo)[PCH], bool jl, const int4 *jtabL,
+                                             const int4 *jtabG, int n, const int *fs, const int *bs,
+                                             uint64_t unsched)
+{
+    if (n <= 32)
+        return jl ? lb2_pairs<PCH, true>(pinfo, jtabL, n, fs, bs, unsched)
+                  : lb2_pairs<PCH, true>(pinfo, jtabG, n, fs, bs, unsched);
+    return jl ? lb2_pairs<PCH, false>(pinfo, jtabL, n, fs, bs, unsched)
+              : lb2_pairs<PCH, false>(pinfo, jtabG, n, fs, bs, unsched);
 }
 
 }  // namespace pbb

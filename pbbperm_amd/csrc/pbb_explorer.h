@@ -617,8 +617,7 @@ struct Explorer {
             r = wave_max(lane < M ? cc + bq : 0);
         } else {
             const uint64_t unsched = rowmask & ~(1ull << cj);
-            r = lds.jtl ? lb2_pairs<PCH>(pinfo, lds.jtabL, n, fs, bs, unsched)
-                        : lb2_pairs<PCH>(pinfo, lds.jtabG, n, fs, bs, unsched);
+            r = lb2_pairs_any<PCH>(pinfo, lds.jtl, lds.jtabL, lds.jtabG, n, fs, bs, unsched);
         }
         __builtin_amdgcn_wave_barrier();
         return r;

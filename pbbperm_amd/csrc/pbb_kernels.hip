@@ -246,8 +246,7 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
         __builtin_amdgcn_wave_barrier();
         int pinfo[Explorer<M, 2>::PCH];
         pair_lanes<Explorer<M, 2>::PCH>(a.I.pair_ab, a.I.npairs, a.I.m, arg, pinfo);
-        int v = lds.jtl ? lb2_pairs<Explorer<M, 2>::PCH>(pinfo, lds.jtabL, a.I.n, fs, bs, um)
-                        : lb2_pairs<Explorer<M, 2>::PCH>(pinfo, lds.jtabG, a.I.n, fs, bs, um);
+        int v = lb2_pairs_any<Explorer<M, 2>::PCH>(pinfo, lds.jtl, lds.jtabL, lds.jtabG, a.I.n, fs, bs, um);
         if (lane == 0) o1[s] = v;
     }
 }
