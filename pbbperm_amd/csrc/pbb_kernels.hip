@@ -264,7 +264,9 @@ hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s)
 {
     const bool lb2 = a.bound_mode != 0 && a.refine == 1;
     const size_t lds = explore_lds_bytes(a.I, lb2);
-    hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (const hipError_t err =
+            hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+        return err;
     if (a.bound_mode == 0)
         hipLaunchKernelGGL((k_explore<M, 0>), dim3(blocks), dim3(64 * PBB_EXPLORE_WAVES), lds, s, a);
     else if (a.refine == 1)
@@ -284,7 +286,9 @@ hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
         return hipSuccess;
     }
     const size_t q = residency_lds_bytes(lds);
-    hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)q);
+    if (const hipError_t err =
+            hipFuncSetAttribute(explore_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)q))
+        return err;
     if (a.bound_mode == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, k_explore<M, 0>, 64 * PBB_EXPLORE_WAVES, q);
     if (a.refine == 1)
@@ -296,7 +300,9 @@ template <int M>
 hipError_t launch_root_m(const KArgs &a, const int *root_perm, hipStream_t s)
 {
     const size_t lds = root_lds_bytes(a.I);
-    hipFuncSetAttribute((const void *)k_bound_root<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (const hipError_t err =
+            hipFuncSetAttribute((const void *)k_bound_root<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+        return err;
     hipLaunchKernelGGL(k_bound_root<M>, dim3(1), dim3(64), lds, s, a, root_perm);
     return hipGetLastError();
 }
@@ -307,7 +313,9 @@ hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perm
 {
     const bool lb2 = which == 2 && jtab_in_lds(a.I.n, a.I.npairs);
     const size_t lds = lds_table_bytes(a.I.n, a.I.m, a.I.npairs, lb2) + 4 * lds_wave_bytes(a.I.n, a.I.m, true);
-    hipFuncSetAttribute((const void *)k_bounds<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (const hipError_t err =
+            hipFuncSetAttribute((const void *)k_bounds<M>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+        return err;
     hipLaunchKernelGGL(k_bounds<M>, dim3((count + 3) / 4), dim3(256), lds, s, a, which, count, perms, l1, l2,
                        arg, o1, o2);
     return hipGetLastError();

@@ -537,7 +537,9 @@ hipError_t launch_pool_m(const KArgs &a, const PoolArgs &p, int groups, int init
                        dim3(256), 0, s, p, init_count);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipFuncSetAttribute(pool_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (const hipError_t err =
+            hipFuncSetAttribute(pool_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+        return err;
     if (a.bound_mode == 0)
         hipLaunchKernelGGL((k_pool<M, 0>), dim3(groups), dim3(64 * p.W), lds, s, a, p);
     else if (a.refine == 1)
@@ -556,7 +558,9 @@ hipError_t pool_occupancy_m(const KArgs &a, const PoolArgs &p, int *groups_per_c
         return hipSuccess;
     }
     const size_t q = residency_lds_bytes(lds);
-    hipFuncSetAttribute(pool_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)q);
+    if (const hipError_t err =
+            hipFuncSetAttribute(pool_kernel<M>(a), hipFuncAttributeMaxDynamicSharedMemorySize, (int)q))
+        return err;
     if (a.bound_mode == 0)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(groups_per_cu, k_pool<M, 0>, 64 * p.W, q);
     if (a.refine == 1)
