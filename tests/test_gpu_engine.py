@@ -200,6 +200,65 @@ def test_budgeted_runs_resume(oracle_mod):
     assert _same_counts(st, ref)
 
 
+@pytest.mark.parametrize("n", [3, 4, 5, 6, 7])
+def test_smallest_instances_all_modes_frozen_ub(oracle_mod, n):
+    """n from 3 (the smallest the engine accepts) up: the root row, the
+    last-line leaf and the l2 - l1 <= 2 LB2 case meet at once."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    for m in (2, 3, 5):
+        ptm = O.synth_ptm(900 + 10 * n + m, n, m)
+        opt = _opt(O, ptm)
+        for (bm, br, rf, jp) in MODES:
+            ref = O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+            best, _, st = _gpu(P, ptm, opt, bm, br, rf, jp, nb=64)
+            assert best == opt and _same_counts(st, ref), (n, m, bm, br, rf, jp)
+
+
+@pytest.mark.parametrize("m,tail,seed,mode", [(5, 12, 77, (0, 1, 0, 0)), (20, 14, 80, (0, 1, 0, 0)),
+                                              (20, 14, 80, (2, 3, 1, 0))])
+def test_largest_row_deep_interval(oracle_mod, m, tail, seed, mode):
+    """n = 63, the longest IVM row a wavefront holds: one interval that fixes a
+    random prefix and spans the last `tail` levels, given to one explorer of
+    64 (the rest take work from it).  With the incumbent frozen at the
+    interval's optimum the counts equal the oracle's (here the unfold down the
+    prefix, bounded at full length, decides them); with an improving
+    incumbent the engine finds that optimum and a schedule realising it
+    (node counts then depend on the exploration order, so only the optimum is
+    compared)."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    n = 63
+    bm, br, rf, jp = mode
+    ptm = O.synth_ptm(seed, n, m)
+    inst = O.Instance(ptm)
+    rng = np.random.default_rng(5)
+    pre = [int(rng.integers(0, n - i)) for i in range(n - tail)]
+    pos = np.array(pre + [0] * tail, np.intc)
+    end = np.array(pre + list(range(tail - 1, -1, -1)), np.intc)
+    bb = O.BB(inst, INT_MAX, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+    bb.init_interval(pos, end)
+    bb.run()
+    opt = bb.best()[0]
+    assert bb.stats().decomposed > 50
+    bb = O.BB(inst, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+    bb.init_interval(pos, end)
+    bb.run()
+    ref = bb.stats()
+    cfg = _cfg(P, bm, br, rf, jp, nb=64)
+    with P.GpuBB(ptm, cfg, ub=opt) as eng:
+        eng.init_from_fac(np.array([0]), pos[None, :], end[None, :])
+        assert eng.next(time_budget=60.0)
+        assert _same_counts(eng.stats(), ref)
+    with P.GpuBB(ptm, cfg) as eng:
+        eng.init_from_fac(np.array([0]), pos[None, :], end[None, :])
+        assert eng.next(time_budget=60.0)
+        best, perm = eng.best()
+        print(f"n=63 m={m} tail={tail} optimum={opt} found={best} dec={eng.stats().decomposed}", flush=True)
+    assert best == opt
+    assert sorted(perm.tolist()) == list(range(n)) and inst.makespan(perm) == opt
+
+
 def test_factoradic_intervals_import(oracle_mod):
     """initFromFac with the full interval cut in pieces: each piece is unfolded
     (bounded, uncounted); counts equal the oracle summed over the pieces."""
