@@ -24,7 +24,9 @@ Three more legs, reported as objects of the same line:
              20-machine requests folded to n <= 16; 600 by default, the
              10,000-request stream with --serve-requests 10000) through the
              resident serving loop and through the plain path, same stream,
-             same arrival times: solves/s and p50/p99 latency.
+             same arrival times: solves/s and p50/p99 latency (with
+             PBB_REPLAY_PROGRESS=1 in the environment both replays report
+             every 500 served requests on stderr).
 Pool parameters come from the autotuner's stored choice
 (pbbperm_amd/tuned.json, python -m pbbperm_amd.tune), or --autotune.
 Every optimum of both legs is compared between the pool and the plain path.

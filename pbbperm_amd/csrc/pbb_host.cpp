@@ -446,6 +446,15 @@ int pbb::check_shape(const int32_t *ptm, int n, int m)
     return PBB_OK;
 }
 
+void pbb::replay_progress(const char *what, int before, int after, int count)
+{
+    static const bool on = getenv("PBB_REPLAY_PROGRESS") != nullptr;
+    if (on && (after / 500 > before / 500 || (after == count && before < count))) {
+        fprintf(stderr, "[%s] %d / %d requests served\n", what, after, count);
+        fflush(stderr);
+    }
+}
+
 int pbb::check_device()
 {
     int count = 0;
@@ -1085,6 +1094,7 @@ extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t 
         for (int j = 0; j < PBB_MAX_JOBS; j++) o.perm[j] = (o.improved && j < e->n) ? perm[j] : -1;
         latency_ms[i] = (tc - arrival_s[i]) * 1e3;
         t_last = tc;
+        replay_progress("plain replay", i, i + 1, count);
     }
     cleanup();
     if (wall_s) *wall_s = count ? t_last - arrival_s[0] : 0.0;

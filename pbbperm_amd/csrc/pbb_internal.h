@@ -13,6 +13,9 @@ int fail(int code, const char *fmt, ...);
 int validate(const pbb_config *c);
 int check_shape(const int32_t *ptm, int n, int m);
 int check_device();
+// Stream replays report on stderr every 500 served requests when the
+// environment sets PBB_REPLAY_PROGRESS (a long stream runs for minutes).
+void replay_progress(const char *what, int before, int after, int count);
 
 }  // namespace pbb
 

@@ -576,6 +576,7 @@ extern "C" int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, con
                 latency_ms[i] = (tc - arrival_s[i]) * 1e3;
                 t_last = tc;
             }
+            replay_progress("pool replay", done, done + k, count);
             done += k;
             progress = true;
         }
