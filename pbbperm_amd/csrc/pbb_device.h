@@ -42,9 +42,7 @@ __device__ __forceinline__ int wrlane(int old, int v, int l)
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-// lane src of v (ds_bpermute: the source lane is taken modulo 64, as __shfl
-// over the whole wavefront would)
-__device__ __forceinline__ int shfl(int v, int src) { return __builtin_amdgcn_ds_bpermute(src << 2, v); }
+__device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, PBB_WAVE); }
 
 // Wavefront reductions on DPP lanes moves (row_shr 1,2,4,8 inside each
 // 16-lane row, row_bcast:15 and row_bcast:31 across rows): the result is
@@ -81,10 +79,8 @@ __device__ __forceinline__ int bcast_high_half(int x)
     return __builtin_amdgcn_permlane32_swap(x, x, false, false)[1];
 }
 
-// misc.h negative() / absolute(): -j-1 is ~j, which has the opposite sign
-// of j, so the flagged value is the smaller of the two and the plain one the larger
-__device__ __forceinline__ int flag_neg(int j) { return min(j, ~j); }
-__device__ __forceinline__ int flag_abs(int j) { return max(j, ~j); }
+__device__ __forceinline__ int flag_neg(int j) { return j < 0 ? j : -j - 1; }  // misc.h negative()
+__device__ __forceinline__ int flag_abs(int j) { return j >= 0 ? j : -j - 1; }  // misc.h absolute()
 
 // Instance tables as seen by the device.
 struct DevInst {
