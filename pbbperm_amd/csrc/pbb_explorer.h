@@ -1039,6 +1039,13 @@ struct Explorer {
         hist = 0;
     }
 
+    // incumbent word loaded by the caller (cost << 32 | explorer)
+    __device__ __forceinline__ void take_best(unsigned long long b)
+    {
+        const int bc = uni((int)(unsigned)(b >> 32));
+        if (bc < local_best) local_best = bc;
+    }
+
     // periodic: node budget, deadline, host interrupt, incumbent refresh
     __device__ __forceinline__ bool check_stop()
     {

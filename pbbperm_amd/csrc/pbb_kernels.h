@@ -24,6 +24,19 @@ enum { PBB_DEV_ERR_RING = 1 };
 #define PBB_POLL_EVERY 16
 #endif
 
+// A busy explorer reloads the shared incumbent every PBB_BEST_EVERY decomposed
+// nodes (power of two; the load is issued before the node's work and applied
+// after it), and an idle one when a donation wakes it, so an improvement found
+// by one explorer prunes the others' subtrees within that many nodes.  With a
+// frozen incumbent nothing changes.  Measured on the 20x5 NEH-incumbent solve
+// (4096 explorers): refreshing only at the periodic check (every 64 nodes,
+// which most explorers of a small tree never reach) decomposed 126-136 k nodes
+// in 5-6 ms; on wake-up and every 64 / 16 / 8 nodes, 12-13 k / 6-10 k / 6 k
+// nodes in 1.1-1.4 ms; the 50x20 explorer is within noise at 16.
+#ifndef PBB_BEST_EVERY
+#define PBB_BEST_EVERY 16
+#endif
+
 // The host's interrupt word lives in host memory, so reading it is a PCIe
 // round trip.  A busy explorer reads it at every PBB_HOST_POLL_CHECKS-th
 // periodic check (one check per 64 decomposed nodes; explorers start at

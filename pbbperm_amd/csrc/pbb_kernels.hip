@@ -71,6 +71,9 @@ __global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                     st = ST_ACTIVE;
                     X.load_state();
+                    // a donated interval is explored under the current incumbent,
+                    // not the one this explorer saw before it went idle
+                    X.take_best(ld_rlx(a.best));
                     break;
                 }
                 if (uni(ld_rlx(&a.ctl->done)) || uni(ld_rlx(&a.ctl->stop)) ||
@@ -88,11 +91,15 @@ __global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_
             rh = ld_rlx(&a.ctl->idle_head);
             rt = ld_rlx(&a.ctl->idle_tail);
         }
+        const bool refresh = (iter & (PBB_BEST_EVERY - 1)) == 0u;
+        unsigned long long bw = ~0ull;
+        if (refresh) bw = ld_rlx(a.best);
         if (!X.step()) {
             if (X.go_idle() && X.lane == 0) st_rel(&a.ctl->done, 1);
             st = ST_IDLE;
             continue;
         }
+        if (refresh) X.take_best(bw);
         if ((++iter & 63u) == 0u && X.check_stop()) {
             X.save_state(ST_ACTIVE);
             X.flush_counters();

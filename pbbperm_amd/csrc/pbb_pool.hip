@@ -436,6 +436,7 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolA
                             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                             st = ST_ACTIVE;
                             X.load_state();
+                            X.take_best(ld_rlx(&S->best));
                             break;
                         }
                         if (uni(ld_rlx(&S->c.done)) || uni(ld_rlx(&p.run->stop)) ||
@@ -453,11 +454,15 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolA
                     rh = ld_rlx(&S->c.idle_head);
                     rt = ld_rlx(&S->c.idle_tail);
                 }
+                const bool refresh = (iter & (PBB_BEST_EVERY - 1)) == 0u;
+                unsigned long long bw = ~0ull;
+                if (refresh) bw = ld_rlx(&S->best);
                 if (!X.step()) {
                     if (X.go_idle()) pool_finish(X, a, S);
                     st = ST_IDLE;
                     continue;
                 }
+                if (refresh) X.take_best(bw);
                 if ((++iter & 63u) == 0u && X.check_stop()) break;
                 if (poll) hungry = X.try_donate(rh, rt);
             }
