@@ -23,8 +23,10 @@ Three more legs, reported as objects of the same line:
   "serving"  a Poisson stream of requests (n ~ U[12,30], m ~ U{5,10,20},
              20-machine requests folded to n <= 16; 600 by default, the
              10,000-request stream with --serve-requests 10000) through the
-             resident serving loop and through the plain path, same stream,
-             same arrival times: solves/s and p50/p99 latency (with
+             resident serving loop, through a 10-machine and a 20-machine
+             pool resident together with each request routed to the smaller
+             one that holds it ("routed"), and through the plain path, same
+             stream, same arrival times: solves/s and p50/p99 latency (with
              PBB_REPLAY_PROGRESS=1 in the environment both replays report
              every 500 served requests on stderr).
 Pool parameters come from the autotuner's stored choice
@@ -250,17 +252,26 @@ def leg_serving(P, PP, args):
     note(f"serving: {N} requests at {rate:.1f}/s through the pool")
     with PP.Pool(cfg, params=best) as pool:
         res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=900.0)
-    note(f"serving: pool done in {wall:.1f} s; plain path")
+    note(f"serving: pool done in {wall:.1f} s; routed pools")
+    rprm = PP.serving_routed_params(best)
+    with PP.Pool(cfg, params=rprm[1]) as p20, PP.Pool(cfg, params=rprm[0]) as p10:
+        rres, rlat, rwall = PP.replay_routed([p20, p10], ptms, ubs, arrivals, time_budget=120.0)
+    note(f"serving: routed done in {rwall:.1f} s; plain path")
     plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=pcfg, time_budget=900.0)
     note(f"serving: plain done in {pwall:.1f} s")
-    ok = all(r.status == "solved" for r in res) and all(r.status == "solved" for r in plain)
+    ok = all(r.status == "solved" for r in res + rres + plain)
     same = all(a.best == b.best for a, b in zip(res, plain))
+    rsame = all(a.best == b.best for a, b in zip(rres, plain))
     return {
         "workload": f"{N} Poisson requests, seeds 10000+, n~U[12,30], m~U{{5,10,20}} (m=20 folded to n<=16), "
                     "LB1 + Johnson LB2, MinMin, NEH incumbents, every request solved to optimality",
         "rate_per_s": rate, "load_vs_plain": args.serve_load,
         "solves_per_s": N / wall, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99), "max_ms": float(np.max(lat)),
         "decomposed": int(sum(r.decomposed for r in res)),
+        "routed": {"pools": [{"machines": q.machines, "max_jobs": q.max_jobs, "groups": q.groups} for q in rprm],
+                   "solves_per_s": N / rwall, "p50_ms": _pct(rlat, 50), "p99_ms": _pct(rlat, 99),
+                   "mean_ms": float(np.mean(rlat)), "max_ms": float(np.max(rlat)), "optima_equal_plain": rsame},
+        "mean_ms": float(np.mean(lat)),
         "plain": {"solves_per_s": N / pwall, "p50_ms": _pct(plat, 50), "p99_ms": _pct(plat, 99),
                   "max_ms": float(np.max(plat)), "explorers_per_instance": best.explorers_per_instance},
         "all_solved": ok, "optima_equal_plain": same,

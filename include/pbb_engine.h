@@ -322,6 +322,22 @@ int pbb_serve_footprint(pbb_pool *p, int n, uint64_t *bytes);
 int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, const int *n, const int *m,
                      const int *ub, const double *arrival_s, double time_budget_s, pbb_result *out,
                      double *latency_ms, double *wall_s);
+/* The same stream through several pools resident on the device together
+ * (no counterpart in the reference): request i goes to the pool with the
+ * fewest machines whose shape holds it (n <= max_jobs, m <= machines; ties go
+ * to the smaller max_jobs), so a 5- or 10-machine request is not padded to
+ * the largest machine count.  1 to 8 distinct pools; a request that no pool
+ * holds fails the call with PBB_E_ARG before anything is submitted.  The
+ * serving kernels start in decreasing order of machines, and each stays
+ * resident until the replay ends, so a later pool only gets the registers
+ * and LDS the earlier ones leave: give every pool an explicit `groups` small
+ * enough that each later pool keeps room for its groups (a group that is
+ * not resident only delays, but a pool with none serves nothing until the
+ * time budget ends the replay).  Measured on one MI355X: 192 groups of a
+ * 20-machine pool started before 512 groups of a 10-machine pool. */
+int pbb_serve_replay_routed(pbb_pool *const *pools, int npools, int count, const int32_t *ptms, const int *n,
+                            const int *m, const int *ub, const double *arrival_s, double time_budget_s,
+                            pbb_result *out, double *latency_ms, double *wall_s);
 /* The same stream on the one-instance engine, one launch per instance in
  * arrival order (first come, first served). */
 int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, const int *n,

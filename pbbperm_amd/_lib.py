@@ -133,6 +133,9 @@ SIGNATURES = {
     "pbb_serve_footprint": (C.c_int, [VP, C.c_int, U64P]),
     "pbb_serve_replay": (C.c_int, [VP, C.c_int, I32P, IP, IP, IP, C.POINTER(C.c_double), C.c_double,
                                    C.POINTER(Result), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "pbb_serve_replay_routed": (C.c_int, [C.POINTER(VP), C.c_int, C.c_int, I32P, IP, IP, IP,
+                                          C.POINTER(C.c_double), C.c_double, C.POINTER(Result),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pbb_plain_replay": (C.c_int, [C.POINTER(Config), C.c_int, I32P, IP, IP, IP, C.POINTER(C.c_double),
                                    C.c_double, C.POINTER(Result), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]),

@@ -522,22 +522,31 @@ extern "C" int pbb_serve_stop(pbb_pool *p)
     return check_run_error(p);
 }
 
-extern "C" int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, const int *n, const int *m,
-                                const int *ub, const double *arrival_s, double time_budget_s, pbb_result *out,
-                                double *latency_ms, double *wall_s)
+// The serving loop of one or more pools sharing the device: request i goes
+// to pools[route[i]]; every pool keeps its own pending queue.
+static int replay_pools(pbb_pool *const *pools, int np, const std::vector<int> &route, int count,
+                        const int32_t *ptms, const int *n, const int *m, const int *ub, const double *arrival_s,
+                        double time_budget_s, pbb_result *out, double *latency_ms, double *wall_s)
 {
-    if (!p || count < 0 || (count > 0 && (!ptms || !n || !m || !ub || !arrival_s || !out || !latency_ms)))
-        return fail(PBB_E_ARG, "bad replay arguments");
     std::vector<size_t> off(count + 1, 0);
-    for (int i = 0; i < count; i++) {
-        int rc = check_shape(ptms + off[i], n[i], m[i]);
-        if (rc) return rc;
-        if (n[i] > p->nmax || m[i] > p->M) return fail(PBB_E_ARG, "instance %d exceeds the pool shape", i);
-        off[i + 1] = off[i] + (size_t)n[i] * m[i];
-    }
+    for (int i = 0; i < count; i++) off[i + 1] = off[i] + (size_t)n[i] * m[i];
     const double budget = time_budget_s > 0 ? time_budget_s : 600.0;
-    int rc = pbb_serve_start(p, budget + 5.0);
-    if (rc) return rc;
+    // Start the pools with the most machines (the most registers per wave)
+    // first.  Their serving kernels stay resident until stopped, so a pool
+    // started later only gets the register space the earlier ones leave:
+    // 512 groups of a 10-machine pool started first hold 2 waves of 168
+    // VGPRs on every SIMD and leave no room for one 228-VGPR wave of a
+    // 20-machine pool, whose requests then wait for the time budget.
+    std::vector<int> order(np);
+    for (int k = 0; k < np; k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return pools[a]->M > pools[b]->M; });
+    for (int j = 0; j < np; j++) {
+        int rc = pbb_serve_start(pools[order[j]], budget + 5.0);
+        if (rc) {
+            for (int i = 0; i < j; i++) pbb_serve_stop(pools[order[i]]);
+            return rc;
+        }
+    }
     for (int i = 0; i < count; i++) {
         latency_ms[i] = -1.0;
         memset(&out[i], 0, sizeof out[i]);
@@ -545,50 +554,109 @@ extern "C" int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, con
         out[i].status = PBB_RES_ABORTED;
         out[i].best = ub[i];
     }
-    std::deque<int> pending;
-    std::vector<pbb_result> got(std::max(1, p->prm.slots));
-    int next = 0, done = 0;
+    std::vector<std::deque<int>> pending(np);
+    int slots = 1;
+    for (int k = 0; k < np; k++) slots = std::max(slots, pools[k]->prm.slots);
+    std::vector<pbb_result> got(slots);
+    int next = 0, done = 0, rc = PBB_OK;
     double t_last = 0.0;
     const auto t0 = Clock::now();
-    while (done < count) {
+    while (done < count && rc == PBB_OK) {
         const double t = secs_since(t0);
         if (t > budget) break;
         bool progress = false;
-        while (next < count && arrival_s[next] <= t) pending.push_back(next++);
-        while (!pending.empty()) {
-            const int i = pending.front();
-            rc = pbb_serve_submit(p, ptms + off[i], n[i], m[i], ub[i], i);
-            if (rc == PBB_E_BUSY) break;
-            if (rc) {
-                pbb_serve_stop(p);
-                return rc;
-            }
-            pending.pop_front();
-            progress = true;
+        while (next < count && arrival_s[next] <= t) {
+            pending[route[next]].push_back(next);
+            next++;
         }
-        int k = 0;
-        pbb_serve_poll(p, got.data(), (int)got.size(), &k);
-        if (k) {
-            const double tc = secs_since(t0);
-            for (int j = 0; j < k; j++) {
-                const int i = got[j].tag;
-                out[i] = got[j];
-                latency_ms[i] = (tc - arrival_s[i]) * 1e3;
-                t_last = tc;
+        for (int k = 0; k < np && rc == PBB_OK; k++) {
+            while (!pending[k].empty()) {
+                const int i = pending[k].front();
+                const int r = pbb_serve_submit(pools[k], ptms + off[i], n[i], m[i], ub[i], i);
+                if (r == PBB_E_BUSY) break;
+                if (r) {
+                    rc = r;
+                    break;
+                }
+                pending[k].pop_front();
+                progress = true;
             }
-            replay_progress("pool replay", done, done + k, count);
-            done += k;
-            progress = true;
+            int got_k = 0;
+            pbb_serve_poll(pools[k], got.data(), (int)got.size(), &got_k);
+            if (got_k) {
+                const double tc = secs_since(t0);
+                for (int j = 0; j < got_k; j++) {
+                    const int i = got[j].tag;
+                    out[i] = got[j];
+                    latency_ms[i] = (tc - arrival_s[i]) * 1e3;
+                    t_last = tc;
+                }
+                replay_progress("pool replay", done, done + got_k, count);
+                done += got_k;
+                progress = true;
+            }
         }
         if (!progress) std::this_thread::yield();
     }
-    int rc2 = pbb_serve_stop(p);
-    // results that completed while stopping
-    int k = 0;
-    pbb_serve_poll(p, got.data(), (int)got.size(), &k);
-    for (int j = 0; j < k; j++) out[got[j].tag] = got[j];
+    int rc2 = PBB_OK;
+    for (int k = 0; k < np; k++) {
+        const int r = pbb_serve_stop(pools[k]);
+        if (r && !rc2) rc2 = r;
+        // results that completed while stopping
+        int got_k = 0;
+        pbb_serve_poll(pools[k], got.data(), (int)got.size(), &got_k);
+        for (int j = 0; j < got_k; j++) out[got[j].tag] = got[j];
+    }
     if (wall_s) *wall_s = count ? t_last - arrival_s[0] : 0.0;
-    return rc2;
+    return rc ? rc : rc2;
+}
+
+extern "C" int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, const int *n, const int *m,
+                                const int *ub, const double *arrival_s, double time_budget_s, pbb_result *out,
+                                double *latency_ms, double *wall_s)
+{
+    if (!p || count < 0 || (count > 0 && (!ptms || !n || !m || !ub || !arrival_s || !out || !latency_ms)))
+        return fail(PBB_E_ARG, "bad replay arguments");
+    size_t off = 0;
+    for (int i = 0; i < count; i++) {
+        int rc = check_shape(ptms + off, n[i], m[i]);
+        if (rc) return rc;
+        if (n[i] > p->nmax || m[i] > p->M) return fail(PBB_E_ARG, "instance %d exceeds the pool shape", i);
+        off += (size_t)n[i] * m[i];
+    }
+    pbb_pool *const pools[1] = {p};
+    return replay_pools(pools, 1, std::vector<int>(count, 0), count, ptms, n, m, ub, arrival_s, time_budget_s,
+                        out, latency_ms, wall_s);
+}
+
+extern "C" int pbb_serve_replay_routed(pbb_pool *const *pools, int npools, int count, const int32_t *ptms,
+                                       const int *n, const int *m, const int *ub, const double *arrival_s,
+                                       double time_budget_s, pbb_result *out, double *latency_ms, double *wall_s)
+{
+    if (!pools || npools < 1 || npools > 8 || count < 0 ||
+        (count > 0 && (!ptms || !n || !m || !ub || !arrival_s || !out || !latency_ms)))
+        return fail(PBB_E_ARG, "bad replay arguments");
+    for (int k = 0; k < npools; k++) {
+        if (!pools[k]) return fail(PBB_E_ARG, "null pool %d", k);
+        for (int j = 0; j < k; j++)
+            if (pools[j] == pools[k]) return fail(PBB_E_ARG, "pool %d repeated", k);
+    }
+    std::vector<int> route(count, -1);
+    size_t off = 0;
+    for (int i = 0; i < count; i++) {
+        int rc = check_shape(ptms + off, n[i], m[i]);
+        if (rc) return rc;
+        off += (size_t)n[i] * m[i];
+        // fewest machines first, then the smallest job limit
+        for (int k = 0; k < npools; k++) {
+            if (n[i] > pools[k]->nmax || m[i] > pools[k]->M) continue;
+            const pbb_pool *r = route[i] < 0 ? nullptr : pools[route[i]];
+            if (!r || pools[k]->M < r->M || (pools[k]->M == r->M && pools[k]->nmax < r->nmax)) route[i] = k;
+        }
+        if (route[i] < 0) return fail(PBB_E_ARG, "instance %d fits no pool", i);
+    }
+    return replay_pools(pools, npools, route, count, ptms, n, m, ub, arrival_s, time_budget_s, out, latency_ms,
+                        wall_s);
 }
 
 // -------------------------------------------------------------- autotuner

@@ -182,6 +182,26 @@ def neh_device(ptms: Sequence[np.ndarray]):
     return [perms[64 * i:64 * i + int(ns[i])].copy() for i in range(len(ptms))], costs[:len(ptms)].copy()
 
 
+def replay_routed(pools: Sequence["Pool"], ptms: Sequence[np.ndarray], ubs: Sequence[int],
+                  arrivals: Sequence[float], time_budget: float = 0.0):
+    """Serve one arrival stream through several pools resident together: each
+    request goes to the pool with the fewest machines that holds it.  The
+    pools' serving kernels start in decreasing order of machines; size each
+    pool's groups so the later ones stay resident beside the earlier ones.
+    Returns (results, latency_ms, wall_s) in request order."""
+    flat, ns, ms = _pack(ptms)
+    ub = np.ascontiguousarray(ubs, dtype=np.int32)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    hs = (C.c_void_p * len(pools))(*[p._h for p in pools])
+    out = (Result * max(1, len(ptms)))()
+    lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
+    wall = C.c_double()
+    check(load().pbb_serve_replay_routed(hs, len(pools), len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns),
+                                         _ip(ms), _ip(ub), _dp(arr), float(time_budget), out, _dp(lat),
+                                         C.byref(wall)))
+    return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
+
+
 def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Sequence[float],
                  config: Optional[Config] = None, time_budget: float = 0.0):
     """The same stream on the one-instance engine, one launch per instance, FCFS."""
@@ -344,6 +364,24 @@ WORKLOADS = {
     "serving": dict(config=_LB12, max_jobs=30, machines=20, slots=256,
                     tune=lambda: serving_stream(120, 30000), objective="replay_mean", load=0.8),
 }
+
+# The serving stream split over two pools resident together (replay_routed):
+# 5- and 10-machine requests on a 10-machine pool, 20-machine requests
+# (n <= 16) on a 20-machine pool.  Group counts from a sweep on the
+# 600-request stream at 30 requests/s on one MI355X (p50 / mean latency,
+# ms): 576+128 groups 13.5 / 52.2, 512+192 11.2 / 49.5, 384+256 21.4 / 157;
+# one 20-machine pool 26.3-26.5 / 59.6-73.5.
+SERVING_ROUTED = (dict(machines=10, max_jobs=30, groups=512),
+                  dict(machines=20, max_jobs=SERVE_NMAX_M20, groups=192))
+
+
+def serving_routed_params(base: PoolParams) -> List[PoolParams]:
+    """PoolParams of the routed serving pools, with base's tuned waves per
+    group, explorers per instance and steal margin."""
+    keep = dict(slots=base.slots, waves_per_group=base.waves_per_group,
+                explorers_per_instance=base.explorers_per_instance, steal_margin=base.steal_margin)
+    return [default_params(**keep, **shape) for shape in SERVING_ROUTED]
+
 
 TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned.json")
 

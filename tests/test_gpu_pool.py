@@ -218,6 +218,44 @@ def test_serving_improving_ub_optimum_exact():
         _check_schedule(P, p, r)
 
 
+def test_routed_serving_matches_fixture():
+    """The 256-request stream through a 5-, a 10- and a 20-machine pool
+    resident together, each request routed to the smallest pool that holds
+    it: every request is solved with the reference's frozen-bound counts,
+    however the stream is split.  A request that fits no pool, a repeated
+    pool or an empty pool list is refused before anything runs."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    fx = _serve_fixture()
+    sp = fx["spec"]
+    inst = fx["instances"]
+    ptms = [P.synth(d["seed"], d["n"], d["m"]) for d in inst]
+    ubs = [d["opt"] for d in inst]
+    assert {p.shape[0] for p in ptms} == {5, 10, 20}
+    cfg = _cfg(P, sp["bound_mode"], sp["branching"], sp["refine"], sp["pairs"])
+    arrivals = PP.poisson_arrivals(len(ptms), rate=4000.0, seed=3)
+    kw = dict(max_jobs=16, slots=32, waves_per_group=4, explorers_per_instance=128)
+    with PP.Pool(cfg, machines=5, groups=64, **kw) as p5, PP.Pool(cfg, machines=20, groups=64, **kw) as p20, \
+            PP.Pool(cfg, machines=10, groups=64, **kw) as p10:
+        res, lat, wall = PP.replay_routed([p20, p5, p10], ptms, ubs, arrivals, time_budget=BUDGET)
+        with pytest.raises(P.PbbError):
+            PP.replay_routed([p5, p10], ptms, ubs, arrivals, time_budget=BUDGET)   # m = 20 fits neither
+        with pytest.raises(P.PbbError):
+            PP.replay_routed([p20, p20], ptms, ubs, arrivals, time_budget=BUDGET)
+        with pytest.raises(P.PbbError):
+            PP.replay_routed([], ptms, ubs, arrivals, time_budget=BUDGET)
+        # the pools serve again after the refusals
+        again, _, _ = PP.replay_routed([p10, p20], ptms[:32], ubs[:32], arrivals[:32], time_budget=BUDGET)
+    for r, d in zip(res, inst):
+        assert r.status == "solved", d
+        assert r.best == d["opt"]
+        assert _counts(r) == d["counts_frozen"], d
+    for r, d in zip(again, inst[:32]):
+        assert r.status == "solved" and _counts(r) == d["counts_frozen"], d
+    assert np.all(lat > 0)
+    print(f"routed 256 in {wall:.3f} s (p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms)")
+
+
 def test_serve_admission_and_poll_api(oracle_mod):
     """Submit/poll by hand: a full pool refuses requests (PBB_E_BUSY) until
     slots free up; the footprint follows explorers x (n^2 + 2n) int32."""

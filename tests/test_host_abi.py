@@ -123,6 +123,35 @@ def test_factoradic_codec():
         F.to_int([3, 0, 0])
 
 
+def test_routed_replay_rejects_bad_pool_lists():
+    """Argument checks of the routed replay that run before any device work:
+    an empty or oversized pool list, a null or repeated pool handle."""
+    import pbbperm_amd as P
+    from pbbperm_amd import _lib
+    lib = P.load()
+    ptm = np.ascontiguousarray(P.synth(0, 6, 3), dtype=np.int32)
+    n = (C.c_int * 1)(6)
+    m = (C.c_int * 1)(3)
+    ub = (C.c_int * 1)(10**6)
+    arr = (C.c_double * 1)(0.0)
+    out = (_lib.Result * 1)()
+    lat = (C.c_double * 1)()
+    wall = C.c_double()
+    fake = C.c_void_p(ptm.ctypes.data)   # never dereferenced: the lists below fail first
+
+    def call(handles):
+        hs = (C.c_void_p * max(1, len(handles)))(*handles)
+        return lib.pbb_serve_replay_routed(hs, len(handles), 1, ptm.ctypes.data_as(_lib.I32P), n, m, ub, arr,
+                                           1.0, out, lat, C.byref(wall))
+
+    assert call([]) == -1                         # PBB_E_ARG
+    assert call([None]) == -1
+    assert "null pool" in lib.pbb_last_error().decode()
+    assert call([fake, fake]) == -1
+    assert "repeated" in lib.pbb_last_error().decode()
+    assert call([fake] * 9) == -1
+
+
 def test_engine_fails_loudly_without_device():
     import torch
     import pbbperm_amd as P
