@@ -209,7 +209,8 @@ struct PoolRes {                  // written once, when the instance retires
 struct PoolQueue {                // request queue: slot ids published by tail
     int tail;
     int closed;
-    int pad[62];
+    int started;                  // set by every workgroup of k_pool that starts
+    int pad[61];
     int ids[1];                   // [qcap]
 };
 

@@ -522,6 +522,22 @@ extern "C" int pbb_serve_stop(pbb_pool *p)
     return check_run_error(p);
 }
 
+// Stop serving on several pools: close every queue first, then wait for
+// each kernel.  A pool whose kernel still has groups waiting for a CU can
+// only finish once the other pools' kernels have exited, so waiting for one
+// pool before the others are closed would hold until their time budgets.
+static int stop_pools(pbb_pool *const *pools, const int *which, int count)
+{
+    for (int j = 0; j < count; j++)
+        if (pools[which[j]]->serving) __atomic_store_n(&pools[which[j]]->sq->closed, 1, __ATOMIC_RELEASE);
+    int rc = PBB_OK;
+    for (int j = 0; j < count; j++) {
+        const int r = pbb_serve_stop(pools[which[j]]);
+        if (r && !rc) rc = r;
+    }
+    return rc;
+}
+
 // The serving loop of one or more pools sharing the device: request i goes
 // to pools[route[i]]; every pool keeps its own pending queue.
 static int replay_pools(pbb_pool *const *pools, int np, const std::vector<int> &route, int count,
@@ -543,8 +559,22 @@ static int replay_pools(pbb_pool *const *pools, int np, const std::vector<int> &
     for (int j = 0; j < np; j++) {
         int rc = pbb_serve_start(pools[order[j]], budget + 5.0);
         if (rc) {
-            for (int i = 0; i < j; i++) pbb_serve_stop(pools[order[i]]);
+            stop_pools(pools, order.data(), j);
             return rc;
+        }
+    }
+    // Every pool needs a resident workgroup: the first group of a serving
+    // kernel to start sets its queue's `started` word.  A pool with none
+    // after a few seconds has no room beside the pools started before it.
+    const auto ts = Clock::now();
+    for (int j = 0; j < np; j++) {
+        const int *flag = &pools[order[j]]->sq->started;
+        while (!__atomic_load_n(flag, __ATOMIC_ACQUIRE) && secs_since(ts) < 5.0) std::this_thread::yield();
+        if (!__atomic_load_n(flag, __ATOMIC_ACQUIRE)) {
+            stop_pools(pools, order.data(), np);
+            return fail(PBB_E_CAPACITY,
+                        "pool %d (%d machines): no workgroup became resident beside the pools started before it; "
+                        "give those pools fewer groups", order[j], pools[order[j]]->M);
         }
     }
     for (int i = 0; i < count; i++) {
@@ -598,10 +628,8 @@ static int replay_pools(pbb_pool *const *pools, int np, const std::vector<int> &
         }
         if (!progress) std::this_thread::yield();
     }
-    int rc2 = PBB_OK;
+    const int rc2 = stop_pools(pools, order.data(), np);
     for (int k = 0; k < np; k++) {
-        const int r = pbb_serve_stop(pools[k]);
-        if (r && !rc2) rc2 = r;
         // results that completed while stopping
         int got_k = 0;
         pbb_serve_poll(pools[k], got.data(), (int)got.size(), &got_k);

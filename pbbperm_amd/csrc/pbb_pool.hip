@@ -345,6 +345,7 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolA
     const int lane = lane_id();
     const bool lb2 = REF == 2;
     unsigned long long t_end = 0ull;
+    if (threadIdx.x == 0) sys_st(&p.q->started, 1);
     if (a.deadline) {
         unsigned long long t0 = wall_clock64();
         if (threadIdx.x == 0) s_pick[0] = (int)(unsigned)t0, s_pick[1] = (int)(unsigned)(t0 >> 32);

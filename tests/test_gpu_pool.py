@@ -256,6 +256,34 @@ def test_routed_serving_matches_fixture():
     print(f"routed 256 in {wall:.3f} s (p50 {np.percentile(lat, 50):.2f} ms, p99 {np.percentile(lat, 99):.2f} ms)")
 
 
+def test_routed_serving_with_a_partly_resident_pool_ends_promptly():
+    """A 10-machine pool sized for the whole device (groups=0), listed
+    before a 20-machine pool: the 20-machine pool's kernel starts first, so
+    some 10-machine groups wait for a CU for the whole replay.  Every
+    request is still solved, and stopping closes both pools before waiting
+    for either, so the call returns without waiting for the kernels' time
+    budget (BUDGET + 5 s)."""
+    import time
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    fx = _serve_fixture()
+    sp = fx["spec"]
+    inst = fx["instances"][:48]
+    ptms = [P.synth(d["seed"], d["n"], d["m"]) for d in inst]
+    cfg = _cfg(P, sp["bound_mode"], sp["branching"], sp["refine"], sp["pairs"])
+    arrivals = PP.poisson_arrivals(len(ptms), rate=2000.0, seed=4)
+    kw = dict(max_jobs=16, slots=32, waves_per_group=4, explorers_per_instance=128)
+    with PP.Pool(cfg, machines=10, groups=0, **kw) as p10, PP.Pool(cfg, machines=20, groups=64, **kw) as p20:
+        assert p10.info()["groups"] > 64
+        t0 = time.time()
+        res, lat, wall = PP.replay_routed([p10, p20], ptms, [d["opt"] for d in inst], arrivals,
+                                          time_budget=BUDGET)
+        took = time.time() - t0
+    for r, d in zip(res, inst):
+        assert r.status == "solved" and _counts(r) == d["counts_frozen"], d
+    assert took < BUDGET / 2, took
+
+
 def test_serve_admission_and_poll_api(oracle_mod):
     """Submit/poll by hand: a full pool refuses requests (PBB_E_BUSY) until
     slots free up; the footprint follows explorers x (n^2 + 2n) int32."""
