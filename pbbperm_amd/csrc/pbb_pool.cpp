@@ -551,7 +551,7 @@ static int replay_pools(pbb_pool *const *pools, int np, const std::vector<int> &
     // first.  Their serving kernels stay resident until stopped, so a pool
     // started later only gets the register space the earlier ones leave:
     // 512 groups of a 10-machine pool started first hold 2 waves of 168
-    // VGPRs on every SIMD and leave no room for one 228-VGPR wave of a
+    // VGPRs on every SIMD and leave no room for one 230-VGPR wave of a
     // 20-machine pool, whose requests then wait for the time budget.
     std::vector<int> order(np);
     for (int k = 0; k < np; k++) order[k] = k;

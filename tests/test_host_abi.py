@@ -152,6 +152,23 @@ def test_routed_replay_rejects_bad_pool_lists():
     assert call([fake] * 9) == -1
 
 
+def test_serving_routed_params_keep_the_tuned_fields():
+    """The routed serving pools: a 10-machine pool for n <= 30 and a
+    20-machine pool for the folded 20-machine requests, both with the base
+    parameters' tuned waves per group, explorers per instance and margin."""
+    import pbbperm_amd.pool as PP
+    base = PP.workload_params("serving", waves_per_group=2, explorers_per_instance=512, steal_margin=4)
+    p10, p20 = PP.serving_routed_params(base)
+    assert (p10.machines, p10.max_jobs, p20.machines, p20.max_jobs) == (10, 30, 20, PP.SERVE_NMAX_M20)
+    assert p10.groups > 0 and p20.groups > 0
+    for q in (p10, p20):
+        assert (q.waves_per_group, q.explorers_per_instance, q.steal_margin, q.slots) == (2, 512, 4, base.slots)
+    # every request of the serving stream fits one of the two pools
+    for ptm in PP.serving_stream(200, 10000):
+        m, n = ptm.shape
+        assert any(m <= q.machines and n <= q.max_jobs for q in (p10, p20)), (n, m)
+
+
 def test_engine_fails_loudly_without_device():
     import torch
     import pbbperm_amd as P
