@@ -318,7 +318,10 @@ int pbb_serve_footprint(pbb_pool *p, int n, uint64_t *bytes);
 
 /* Replay an arrival stream through the serving loop: request i (as in
  * pbb_pool_solve) arrives at arrival_s[i] seconds; latency_ms[i] = completion
- * seen by the host - arrival.  *wall_s = first arrival to last completion. */
+ * seen by the host - arrival, or -1 for a request not completed within
+ * time_budget_s (out[i].status then says whether it was solved while the
+ * pool stopped, aborted, or never admitted).  *wall_s = first arrival to last
+ * completion within the budget (0 when none completed). */
 int pbb_serve_replay(pbb_pool *p, int count, const int32_t *ptms, const int *n, const int *m,
                      const int *ub, const double *arrival_s, double time_budget_s, pbb_result *out,
                      double *latency_ms, double *wall_s);

@@ -630,12 +630,12 @@ static int replay_pools(pbb_pool *const *pools, int np, const std::vector<int> &
     }
     const int rc2 = stop_pools(pools, order.data(), np);
     for (int k = 0; k < np; k++) {
-        // results that completed while stopping
+        // results that completed while stopping keep latency -1
         int got_k = 0;
         pbb_serve_poll(pools[k], got.data(), (int)got.size(), &got_k);
         for (int j = 0; j < got_k; j++) out[got[j].tag] = got[j];
     }
-    if (wall_s) *wall_s = count ? t_last - arrival_s[0] : 0.0;
+    if (wall_s) *wall_s = done > 0 ? t_last - arrival_s[0] : 0.0;
     return rc ? rc : rc2;
 }
 
