@@ -253,12 +253,15 @@ extern "C" int pbb_instance_synth_shape(uint64_t seed, int nlo, int nhi, const i
 
 extern "C" int pbb_instance_read_vrf(const char *path, int *n_out, int *m_out, int32_t *ptm, int cap)
 {
+    if (!path || !n_out || !m_out || !ptm || cap < 0) return fail(PBB_E_ARG, "bad VRF arguments");
     std::ifstream f(path);
     if (!f) return fail(PBB_E_ARG, "cannot open %s", path);
     int n = 0, m = 0;
     f >> n >> m;
     if (n < 1 || m < 1) return fail(PBB_E_ARG, "bad VRF header in %s", path);
-    if (n * m > cap) return fail(PBB_E_CAPACITY, "ptm buffer too small (%d < %d)", cap, n * m);
+    // the product in 64 bits: a header such as "65536 65536" wraps an int product
+    if ((long long)n * m > cap)
+        return fail(PBB_E_CAPACITY, "ptm buffer too small (%d < %lld)", cap, (long long)n * m);
     // job-major (machine, time) pairs, transposed on read (instance_vrf.cpp:50-64)
     for (int j = 0; j < n; j++)
         for (int k = 0; k < m; k++) {
@@ -269,6 +272,15 @@ extern "C" int pbb_instance_read_vrf(const char *path, int *n_out, int *m_out, i
     *n_out = n;
     *m_out = m;
     return PBB_OK;
+}
+
+// every job id 0..n-1 exactly once (the ids index the time matrix)
+static bool is_perm(const int *p, int n)
+{
+    std::vector<char> seen(n, 0);
+    for (int i = 0; i < n; i++)
+        if (p[i] < 0 || p[i] >= n || seen[p[i]]++) return false;
+    return true;
 }
 
 static int host_partial_makespan(const int32_t *ptm, int n, int m, const int *seq, int len)
@@ -285,6 +297,7 @@ static int host_partial_makespan(const int32_t *ptm, int n, int m, const int *se
 extern "C" int pbb_makespan_host(const int32_t *ptm, int n, int m, const int *perm, int *cost)
 {
     if (!ptm || !perm || !cost || n < 1 || m < 1) return fail(PBB_E_ARG, "bad makespan args");
+    if (!is_perm(perm, n)) return fail(PBB_E_ARG, "makespan: not a permutation of 0..%d", n - 1);
     *cost = host_partial_makespan(ptm, n, m, perm, n);
     return PBB_OK;
 }
@@ -295,6 +308,7 @@ extern "C" int pbb_makespan_host(const int32_t *ptm, int n, int m, const int *pe
 extern "C" int pbb_neh(const int32_t *ptm, int n, int m, int *perm, int *cost)
 {
     if (!ptm || !perm || !cost || n < 2 || m < 1) return fail(PBB_E_ARG, "bad NEH args");
+    if (!is_perm(perm, n)) return fail(PBB_E_ARG, "NEH: start order is not a permutation of 0..%d", n - 1);
     std::vector<int> tot(n, 0);
     for (int j = 0; j < n; j++)
         for (int k = 0; k < m; k++) tot[j] += ptm[k * n + j];

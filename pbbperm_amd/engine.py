@@ -92,7 +92,9 @@ def read_vrf(path: str) -> np.ndarray:
 def neh(ptm: np.ndarray, start: Optional[Sequence[int]] = None) -> Tuple[int, np.ndarray]:
     ptm = np.ascontiguousarray(ptm, dtype=np.int32)
     m, n = ptm.shape
-    perm = np.arange(n, dtype=np.intc) if start is None else np.ascontiguousarray(start, np.intc)
+    perm = np.arange(n, dtype=np.intc) if start is None else np.array(start, dtype=np.intc)
+    if perm.shape != (n,):
+        raise ValueError(f"NEH start order of length {perm.size}, expected {n}")
     cost = C.c_int()
     check(load().pbb_neh(_i32(ptm), n, m, _ip(perm), C.byref(cost)))
     return cost.value, perm
@@ -102,6 +104,8 @@ def makespan(ptm: np.ndarray, perm: Sequence[int]) -> int:
     ptm = np.ascontiguousarray(ptm, dtype=np.int32)
     m, n = ptm.shape
     p = np.ascontiguousarray(perm, np.intc)
+    if p.shape != (n,):
+        raise ValueError(f"permutation of length {p.size}, expected {n}")
     cost = C.c_int()
     check(load().pbb_makespan_host(_i32(ptm), n, m, _ip(p), C.byref(cost)))
     return cost.value

@@ -107,6 +107,35 @@ def test_vrf_reader(tmp_path):
     assert np.array_equal(P.read_vrf(str(path)), ptm)
 
 
+def test_host_entry_points_reject_bad_inputs(tmp_path):
+    """Inputs that would index out of range are refused: a VRF header whose
+    n*m overflows 32 bits, job ids outside 0..n-1 or repeated, short arrays."""
+    import pbbperm_amd as P
+    from pbbperm_amd import _lib
+    big = tmp_path / "big.txt"
+    big.write_text("65536 65536\n0 1 1 1\n")
+    with pytest.raises(P.PbbError):
+        P.read_vrf(str(big))
+    lib = P.load()
+    n_, m_ = C.c_int(), C.c_int()
+    buf = (C.c_int32 * 4)()
+    assert lib.pbb_instance_read_vrf(str(big).encode(), C.byref(n_), C.byref(m_), buf, 4) == -4  # PBB_E_CAPACITY
+    assert lib.pbb_instance_read_vrf(None, C.byref(n_), C.byref(m_), buf, 4) == -1
+    ptm = P.synth(5, 6, 3)
+    for bad in ([0, 0, 1, 2, 3, 4], [0, 1, 2, 3, 4, 6], [0, 1, 2, 3, 4, -1]):
+        with pytest.raises(P.PbbError):
+            P.makespan(ptm, bad)
+        with pytest.raises(P.PbbError):
+            P.neh(ptm, start=bad)
+    with pytest.raises(ValueError):
+        P.makespan(ptm, [0, 1, 2])
+    with pytest.raises(ValueError):
+        P.neh(ptm, start=[0, 1])
+    # a valid start order other than the identity is still accepted
+    c, perm = P.neh(ptm, start=[5, 4, 3, 2, 1, 0])
+    assert sorted(perm.tolist()) == list(range(6)) and P.makespan(ptm, perm) == c
+
+
 def test_factoradic_codec():
     from pbbperm_amd import factoradic as F
     for n in (3, 7, 20, 50):
