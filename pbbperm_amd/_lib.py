@@ -179,6 +179,16 @@ def load():
     return _lib
 
 
+def sized(values, dtype, size: int, what: str):
+    """values as a contiguous array of exactly `size` elements: the C side
+    reads that many, so a shorter array is refused here."""
+    import numpy as np
+    a = np.ascontiguousarray(values, dtype=dtype)
+    if a.size != size:
+        raise ValueError(f"{what}: {a.size} values, expected {size}")
+    return a
+
+
 def check(rc: int) -> int:
     if rc != 0:
         raise PbbError(rc, load().pbb_last_error().decode())

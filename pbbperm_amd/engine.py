@@ -212,12 +212,12 @@ class GpuBB:
         self.m, self.n = ptm.shape
         self.ptm = ptm
         self.config = config if config is not None else default_config()
-        up = None if ub_perm is None else np.ascontiguousarray(ub_perm, np.intc)
+        up = None if ub_perm is None else _lib.sized(ub_perm, np.intc, self.n, "ub_perm")
+        r = None if root is None else _lib.sized(root, np.intc, self.n, "root order")
         self._h = C.c_void_p()
         check(load().pbb_create(_i32(ptm), self.n, self.m, C.byref(self.config), int(ub),
                                 None if up is None else _ip(up), C.byref(self._h)))
-        if root is not None:
-            r = np.ascontiguousarray(root, np.intc)
+        if r is not None:
             check(load().pbb_set_root(self._h, _ip(r)))
 
     def close(self):
@@ -282,7 +282,7 @@ class GpuBB:
         return cost.value, perm
 
     def set_best(self, cost: int, perm: Optional[Sequence[int]] = None):
-        p = None if perm is None else np.ascontiguousarray(perm, np.intc)
+        p = None if perm is None else _lib.sized(perm, np.intc, self.n, "incumbent permutation")
         check(load().pbb_set_incumbent(self._h, int(cost), None if p is None else _ip(p)))
 
     def stats(self) -> SolveStats:
@@ -341,9 +341,9 @@ def checkpoint_format(n: int, ids, pos, end, root_cost: int, root_perm, simple_b
     for i in range(n):
         c.root_perm[i] = int(root_perm[i])
         c.best_perm[i] = int(best_perm[i])
-    ids = np.ascontiguousarray(ids, dtype=np.intc)
-    pos = np.ascontiguousarray(pos, dtype=np.intc).reshape(-1)
-    end = np.ascontiguousarray(end, dtype=np.intc).reshape(-1)
+    ids = np.ascontiguousarray(ids, dtype=np.intc).reshape(-1)
+    pos = _lib.sized(pos, np.intc, ids.size * n, "pos digits").reshape(-1)
+    end = _lib.sized(end, np.intc, ids.size * n, "end digits").reshape(-1)
     c.nb = len(ids)
     ln = C.c_int()
     check(load().pbb_checkpoint_format(C.byref(c), _ip(ids), _ip(pos), _ip(end), None, 0, C.byref(ln)))

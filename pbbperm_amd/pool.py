@@ -113,8 +113,8 @@ class Pool:
               time_budget: float = 0.0) -> Tuple[List[PoolResult], float]:
         """Solve len(ptms) <= slots instances in one graph launch."""
         flat, ns, ms = _pack(ptms)
-        ub = np.ascontiguousarray(ubs, dtype=np.int32)
-        tg = None if tags is None else np.ascontiguousarray(tags, dtype=np.int32)
+        ub = _lib.sized(ubs, np.int32, len(ptms), "ubs")
+        tg = None if tags is None else _lib.sized(tags, np.int32, len(ptms), "tags")
         out = (Result * max(1, len(ptms)))()
         kms = C.c_double()
         check(self._lib.pbb_pool_solve(self._h, len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms),
@@ -159,8 +159,8 @@ class Pool:
                time_budget: float = 0.0):
         """Serve an arrival stream; returns (results, latency_ms, wall_s)."""
         flat, ns, ms = _pack(ptms)
-        ub = np.ascontiguousarray(ubs, dtype=np.int32)
-        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        ub = _lib.sized(ubs, np.int32, len(ptms), "ubs")
+        arr = _lib.sized(arrivals, np.float64, len(ptms), "arrivals")
         out = (Result * max(1, len(ptms)))()
         lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
         wall = C.c_double()
@@ -190,8 +190,8 @@ def replay_routed(pools: Sequence["Pool"], ptms: Sequence[np.ndarray], ubs: Sequ
     pool's groups so the later ones stay resident beside the earlier ones.
     Returns (results, latency_ms, wall_s) in request order."""
     flat, ns, ms = _pack(ptms)
-    ub = np.ascontiguousarray(ubs, dtype=np.int32)
-    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    ub = _lib.sized(ubs, np.int32, len(ptms), "ubs")
+    arr = _lib.sized(arrivals, np.float64, len(ptms), "arrivals")
     hs = (C.c_void_p * len(pools))(*[p._h for p in pools])
     out = (Result * max(1, len(ptms)))()
     lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
@@ -207,8 +207,8 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
     """The same stream on the one-instance engine, one launch per instance, FCFS."""
     cfg = config if config is not None else default_config()
     flat, ns, ms = _pack(ptms)
-    ub = np.ascontiguousarray(ubs, dtype=np.int32)
-    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    ub = _lib.sized(ubs, np.int32, len(ptms), "ubs")
+    arr = _lib.sized(arrivals, np.float64, len(ptms), "arrivals")
     out = (Result * max(1, len(ptms)))()
     lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
     wall = C.c_double()
@@ -296,7 +296,7 @@ def autotune_native(ptms: Sequence[np.ndarray], ubs: Sequence[int], config: Opti
     cfg = config if config is not None else default_config()
     b = base if base is not None else default_params()
     flat, ns, ms = _pack(ptms)
-    ub = np.ascontiguousarray(ubs, dtype=np.int32)
+    ub = _lib.sized(ubs, np.int32, len(ptms), "ubs")
     best = PoolParams()
     trials = np.zeros(4 * max_trials, dtype=np.float64)
     nt = C.c_int()

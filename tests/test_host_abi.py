@@ -136,6 +136,23 @@ def test_host_entry_points_reject_bad_inputs(tmp_path):
     assert sorted(perm.tolist()) == list(range(6)) and P.makespan(ptm, perm) == c
 
 
+def test_wrappers_refuse_short_arrays():
+    """The C side reads n (or count) values from every array it is given;
+    the wrappers refuse arrays of another length before calling it."""
+    import pbbperm_amd as P
+    from pbbperm_amd import _lib
+    assert _lib.sized([1, 2, 3], np.intc, 3, "x").dtype == np.intc
+    with pytest.raises(ValueError):
+        _lib.sized([1, 2], np.intc, 3, "x")
+    ptm = P.synth(2, 6, 3)
+    with pytest.raises(ValueError):
+        P.GpuBB(ptm, ub=100, ub_perm=[0, 1, 2])
+    with pytest.raises(ValueError):
+        P.GpuBB(ptm, root=[0, 1, 2, 3, 4])
+    with pytest.raises(ValueError):
+        P.checkpoint_format(6, [0, 1], np.zeros(6, np.intc), np.zeros(12, np.intc), 10, range(6), 0, 10, range(6))
+
+
 def test_factoradic_codec():
     from pbbperm_amd import factoradic as F
     for n in (3, 7, 20, 50):
