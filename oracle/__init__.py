@@ -339,12 +339,27 @@ def ref_bounds(ptm: np.ndarray, nsamples: int, seed: int, pairs: int = 0, tmpdir
 
 
 def ref_neh(ptm: np.ndarray, tmpdir="/tmp/pbb_ref"):
-    path = _write_tmp(ptm, tmpdir)
+    """The reference's fastNEH (initialSort + runNEH, identity start) with its
+    uninitialised tail cell zeroed by the harness: (cost, order)."""
+    return ref_neh_many([ptm], tmpdir)[0]
+
+
+def ref_neh_many(ptms: Sequence[np.ndarray], tmpdir="/tmp/pbb_ref"):
+    """ref_neh over many instances in one harness process."""
+    os.makedirs(tmpdir, exist_ok=True)
+    path = os.path.join(tmpdir, f"neh_{os.getpid()}_{len(ptms)}.txt")
+    with open(path, "w") as f:
+        for p in ptms:
+            f.write(ptm_text(p))
     out = subprocess.run([REF_HARNESS, "neh", path], check=True, capture_output=True,
                          text=True).stdout
     os.unlink(path)
-    v = [int(x) for x in out.split()]
-    return v[0], v[1:]
+    res = []
+    for line in out.strip().splitlines():
+        v = [int(x) for x in line.split()]
+        res.append((v[0], v[1:]))
+    assert len(res) == len(ptms), (len(res), len(ptms))
+    return res
 
 
 def ref_taillard(ta_id: int) -> np.ndarray:

@@ -143,19 +143,42 @@ static int cmd_bounds(int argc, char **argv)
     return 0;
 }
 
+// fastNEH::findBestPos reads tail[k][len] (heuristic/src/fastNEH.cpp:159),
+// a cell computeTails(perm, len) never writes (:104-124): as shipped the
+// value at the last insertion position depends on whatever malloc returned.
+// The harness zeroes every tail row after construction (tail is a public
+// member, heuristic/include/fastNEH.h:17-19).  Step k writes cells 0..k-1
+// only, so cell k still holds that zero when step k reads it, and the
+// position-k candidate becomes the makespan of appending the job -- the
+// reference's own code with its undefined read made defined.
+// Input: one or more instances, each "n m" + the m x n times, read until EOF;
+// one output line per instance: the cost then the order.
 static int cmd_neh(int argc, char **argv)
 {
-    ptm_instance inst(read_file(argv[2]));
-    int n = inst.size;
-    fastNEH neh(&inst);
-    std::vector<int> perm(n);
-    for (int i = 0; i < n; i++) perm[i] = i;
-    int cost = 0;
-    neh.initialSort(perm.data());
-    neh.runNEH(perm.data(), cost);
-    printf("%d", cost);
-    for (int i = 0; i < n; i++) printf(" %d", perm[i]);
-    printf("\n");
+    std::stringstream all(read_file(argv[2]));
+    while (true) {
+        int n = 0, m = 0;
+        if (!(all >> n >> m) || n < 2 || m < 1) break;
+        std::stringstream one;
+        one << n << " " << m << "\n";
+        for (int i = 0; i < n * m; i++) {
+            int v = 0;
+            all >> v;
+            one << v << (i % n == n - 1 ? "\n" : " ");
+        }
+        ptm_instance inst(one.str());
+        fastNEH neh(&inst);
+        for (int k = 0; k < neh.nbMachines; k++)
+            for (int j = 0; j < neh.nbJob; j++) neh.tail[k][j] = 0;
+        std::vector<int> perm(n);
+        for (int i = 0; i < n; i++) perm[i] = i;
+        int cost = 0;
+        neh.initialSort(perm.data());
+        neh.runNEH(perm.data(), cost);
+        printf("%d", cost);
+        for (int i = 0; i < n; i++) printf(" %d", perm[i]);
+        printf("\n");
+    }
     return 0;
 }
 

@@ -11,8 +11,16 @@ tree exists.  The fixtures are plain data (inputs and expected outputs):
   solve_cases.json            optimum / decomposed / leaves / LB1 bounds /
                               refinement bounds of single-thread solves
   taillard_ptm.json           sha256 of the ta001..ta120 processing times
+  neh_cases.json              the reference fastNEH (initialSort + runNEH
+                              from the identity order, its uninitialised
+                              tail cell zeroed by the harness) on the 20x5
+                              seed-0 instance, the 512 batched instances,
+                              the 256-request serving stream, one random
+                              instance per n = 3..63 and instances built
+                              to tie: equal job totals and equal insertion
+                              makespans (explicit times)
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [bounds|solve|taillard|neh ...]
 """
 from __future__ import annotations
 
@@ -58,9 +66,64 @@ def ref_bounds_fixture(seed, n, m, count):
     return path
 
 
+def tie_instances():
+    """Instances whose NEH runs through ties: job totals that repeat (the
+    stable initial sort) and insertion positions of equal makespan (the
+    first-minimum rule), from a seeded stream.  Returned with their times."""
+    rng = np.random.default_rng(4711)
+    out = []
+    for i in range(48):
+        n = int(rng.integers(3, 31))
+        m = int(rng.integers(2, 11))
+        kind = i % 4
+        if kind == 0:    # every time equal: all totals and all positions tie
+            ptm = np.full((m, n), int(rng.integers(1, 100)), dtype=np.int32)
+        elif kind == 1:  # times in {1, 2}
+            ptm = rng.integers(1, 3, size=(m, n)).astype(np.int32)
+        elif kind == 2:  # columns that are machine permutations of one job: equal totals
+            base = rng.integers(1, 100, size=m)
+            ptm = np.stack([rng.permutation(base) for _ in range(n)], axis=1).astype(np.int32)
+        else:            # a few distinct jobs, each repeated
+            kinds = rng.integers(1, 100, size=(m, max(1, n // 4)))
+            ptm = kinds[:, rng.integers(0, kinds.shape[1], size=n)].astype(np.int32)
+        out.append(np.ascontiguousarray(ptm))
+    return out
+
+
+def neh_fixture():
+    cases = []  # (kind, seed, ptm, store_ptm)
+    cases.append(("c1_20x5", 0, O.synth_ptm(0, 20, 5), False))
+    for seed in range(100, 612):
+        n, m = O.synth_shape(seed, 10, 22, (5, 10))
+        cases.append(("batch_c4", seed, O.synth_ptm(seed, n, m), False))
+    for seed in range(20000, 20256):
+        n, m = O.synth_shape(seed, 12, 16, (5, 10, 20))
+        cases.append(("serve_small", seed, O.synth_ptm(seed, n, m), False))
+    for n in range(3, 64):
+        m = 2 + (7 * n) % 19
+        cases.append(("random", 30000 + n, O.synth_ptm(30000 + n, n, m), False))
+    for i, ptm in enumerate(tie_instances()):
+        cases.append(("ties", i, ptm, True))
+    got = O.ref_neh_many([c[2] for c in cases])
+    rows = []
+    for (kind, seed, ptm, store), (cost, perm) in zip(cases, got):
+        d = dict(kind=kind, seed=seed, n=int(ptm.shape[1]), m=int(ptm.shape[0]), cost=cost, perm=perm)
+        if store:
+            d["ptm"] = ptm.tolist()
+        rows.append(d)
+    with open(os.path.join(HERE, "neh_cases.json"), "w") as f:
+        json.dump(rows, f, separators=(",", ":"))
+    print("neh cases", len(rows), flush=True)
+
+
 def main():
     if not O.have_ref():
         O.build(ref=True)
+    which = set(sys.argv[1:]) or {"bounds", "solve", "taillard", "neh"}
+    if "neh" in which:
+        neh_fixture()
+    if not which & {"bounds", "solve", "taillard"}:
+        return
     for seed, n, m, count in BOUND_SHAPES:
         print("bounds", seed, n, m, ref_bounds_fixture(seed, n, m, count), flush=True)
 

@@ -384,6 +384,22 @@ def test_device_neh_matches_host_neh():
         assert P.makespan(p, o) == hc
 
 
+def test_device_neh_matches_reference_fixture(oracle_mod):
+    """pbb_neh_device gives the reference fastNEH's order and makespan on the
+    committed fixture (20x5, the 512 batched instances, the serving stream,
+    n = 3..63 and instances built to tie)."""
+    import pbbperm_amd.pool as PP
+    with open(os.path.join(HERE, "golden", "neh_cases.json")) as f:
+        cases = json.load(f)
+    ptms = []
+    for c in cases:
+        p = np.asarray(c["ptm"], dtype=np.int32) if "ptm" in c else oracle_mod.synth_ptm(c["seed"], c["n"], c["m"])
+        ptms.append(np.ascontiguousarray(p))
+    orders, costs = PP.neh_device(ptms)
+    for c, o, cost in zip(cases, orders, costs):
+        assert (int(cost), list(o)) == (c["cost"], c["perm"]), (c["kind"], c["seed"])
+
+
 def test_pool_computes_neh_at_admission(oracle_mod):
     """Requests with ub = UB_NEH: the pool's NEH incumbent then the search;
     the optimum is exact and the returned schedule realises it."""

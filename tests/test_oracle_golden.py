@@ -70,3 +70,72 @@ def test_taillard_known_optima(oracle_mod, ta):
     r = oracle_mod.solve(ptm, TAILLARD_OPT[ta] + 1, bound_mode=0, branching=1)
     assert r.finished and r.best == TAILLARD_OPT[ta]
     assert oracle_mod.Instance(ptm).makespan(r.perm) == r.best
+
+
+def _neh_cases(oracle_mod):
+    with open(os.path.join(GOLDEN, "neh_cases.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        if "ptm" in c:
+            ptm = np.asarray(c["ptm"], dtype=np.int32)
+        else:
+            ptm = oracle_mod.synth_ptm(c["seed"], c["n"], c["m"])
+        yield c, np.ascontiguousarray(ptm)
+
+
+def test_oracle_neh_matches_reference(oracle_mod):
+    """orc_neh gives the reference fastNEH's order and makespan on every
+    fixture case: the 20x5 instance, the 512 batched instances, the serving
+    stream, n = 3..63, and instances built to tie (equal totals, equal
+    insertion makespans)."""
+    kinds = set()
+    for c, ptm in _neh_cases(oracle_mod):
+        cost, perm = oracle_mod.Instance(ptm).neh()
+        assert (cost, list(perm)) == (c["cost"], c["perm"]), (c["kind"], c["seed"])
+        kinds.add(c["kind"])
+    assert kinds == {"c1_20x5", "batch_c4", "serve_small", "random", "ties"}
+
+
+def _neh_variant(ptm, first_min=True, stable=True, keep_two_if_not_worse=False):
+    """A small NEH written out in Python with one rule switched, to show the
+    fixture separates the rules it pins (the reference's rules are
+    first_min=True, stable=True, keep_two_if_not_worse=False)."""
+    m, n = ptm.shape
+
+    def mk(seq):
+        c = [0] * m
+        for j in seq:
+            c[0] += int(ptm[0, j])
+            for k in range(1, m):
+                c[k] = max(c[k - 1], c[k]) + int(ptm[k, j])
+        return c[-1]
+
+    tot = ptm.sum(axis=0)
+    order = sorted(range(n), key=lambda j: (-int(tot[j]), j if stable else -j))
+    a, b = order[0], order[1]
+    keep = mk([a, b]) <= mk([b, a]) if keep_two_if_not_worse else mk([a, b]) < mk([b, a])
+    seq = [a, b] if keep else [b, a]
+    for job in order[2:]:
+        costs = [mk(seq[:p] + [job] + seq[p:]) for p in range(len(seq) + 1)]
+        best = min(costs)
+        pos = costs.index(best) if first_min else len(costs) - 1 - costs[::-1].index(best)
+        seq.insert(pos, job)
+    return mk(seq), seq
+
+
+def test_neh_fixture_separates_tie_rules(oracle_mod):
+    """The tie cases pin each tie rule: switching any one of them changes the
+    order on some fixture instance, while the rules as stated reproduce all."""
+    cases = [(c, p) for c, p in _neh_cases(oracle_mod) if c["kind"] in ("ties", "c1_20x5")]
+    for c, ptm in cases:
+        assert _neh_variant(ptm) == (c["cost"], c["perm"])
+    for kw in (dict(first_min=False), dict(stable=False), dict(keep_two_if_not_worse=True)):
+        assert any(_neh_variant(ptm, **kw)[1] != c["perm"] for c, ptm in cases), kw
+
+
+def test_host_neh_matches_reference(oracle_mod):
+    """The product's host NEH (pbb_neh) on the same fixture."""
+    import pbbperm_amd as P
+    for c, ptm in _neh_cases(oracle_mod):
+        cost, perm = P.neh(ptm)
+        assert (cost, list(perm)) == (c["cost"], c["perm"]), (c["kind"], c["seed"])
