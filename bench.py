@@ -20,21 +20,28 @@ Three more legs, reported as objects of the same line:
              against the plain one-instance-per-launch engine on the same
              instances, with the reference's own code on the host cores
              beside it;
-  "serving"  a Poisson stream of requests (n ~ U[12,30], m ~ U{5,10,20},
-             20-machine requests folded to n <= 16; 600 by default, the
-             10,000-request stream with --serve-requests 10000) through the
-             resident serving loop, through a 10-machine and a 20-machine
-             pool resident together with each request routed to the smaller
-             one that holds it ("routed"), and through the plain path, same
-             stream, same arrival times: solves/s and p50/p99 latency (with
-             PBB_REPLAY_PROGRESS=1 in the environment both replays report
-             every 500 served requests on stderr).
+  "serving"  a Poisson stream of requests (seeds 10000+, n ~ U[12,30],
+             m ~ U{5,10,20}; --serve-requests at --serve-rate per second,
+             each stopped --serve-budget seconds after its admission and then
+             counted late) through the resident serving loop, through a
+             10-machine and a 20-machine pool resident together with each
+             request routed to the smaller one that holds it ("routed"), and
+             through the plain path, same stream, same arrival times, same
+             per-request budget: solves/s, p50/p99 over solved requests, and
+             p99 with the late ones counted (with PBB_REPLAY_PROGRESS=1 in the
+             environment the replays report every 500 served requests on
+             stderr).
+Two legs run only when named: "treesizes" (the serving stream's tree-size
+histogram) and "long" (the headline workload for --long-seconds, step by
+step).
 Pool parameters come from the autotuner's stored choice
 (pbbperm_amd/tuned.json, python -m pbbperm_amd.tune), or --autotune.
-Every optimum of both legs is compared between the pool and the plain path.
+Every optimum of both legs is compared between the pool and the plain path
+(for the serving stream: every request both solved).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--node-budget B]
-                  [--legs nodes,batched,serving] [--serve-requests N]
+                  [--legs nodes,small,batched,serving,treesizes,long]
+                  [--serve-requests N] [--serve-rate R] [--serve-budget S]
 """
 from __future__ import annotations
 
@@ -55,6 +62,14 @@ PEAK_HBM_GBS = 8000.0
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
 # plain 256 MiB device copy): profiles/k_explore_50x20_lb1_pmc.md
 HBM_BYTES_PER_NODE = 0.80
+# HBM bytes of one k_pool<10,2> launch of the 512-instance batch, from the
+# same two counters and factors (None until measured)
+BATCH_HBM_BYTES = None
+# The serving stream of the default run: requests, Poisson rate and the
+# per-request time budget after admission
+SERVE_REQUESTS = 600
+SERVE_RATE = 20.0
+SERVE_BUDGET_S = 10.0
 
 
 def lb1_ops(depth: int, n: int, m: int) -> int:
@@ -65,39 +80,143 @@ def lb1_ops(depth: int, n: int, m: int) -> int:
     return (2 * m - 1) * depth + m * k + 2 * k * (5 * m - 3)
 
 
+def lb2_ops(depth: int, n: int, m: int, pairs: int) -> int:
+    """Algorithmic int32 ops of one Johnson LB2 refinement of a child of a
+    node at depth D (the child has D+1 fixed jobs, k' = n - D - 1 unscheduled):
+    per machine pair a two-machine pass over the k' jobs plus the front/back
+    offsets, P(4k' + 4), and the child's front and back, (2m-1)(D+1)
+    (bound_fsp_strong.cpp:50-162, :441-560)."""
+    kp = n - depth - 1
+    return pairs * (4 * kp + 4) + (2 * m - 1) * (depth + 1)
+
+
+def pool_ops(ptms, dec_histo, ref_histo, pairs_of_m) -> float:
+    """Algorithmic ops of a pool batch from its per-instance depth
+    histograms: interior nodes (depths 1..n-2) x W1 plus refinements x W2."""
+    ops = 0.0
+    for p, hd, hr in zip(ptms, dec_histo, ref_histo):
+        m, n = p.shape
+        P = pairs_of_m(m)
+        for d in range(1, n - 1):
+            ops += float(hd[d]) * lb1_ops(d, n, m) + float(hr[d]) * lb2_ops(d, n, m, P)
+    return ops
+
+
+def host_cpus():
+    """What the CPU baseline runs on: the lscpu model, sockets, physical
+    cores, hardware threads and SMT state of the host, and the CPUs this
+    process may use (its affinity set, at most 16: the host's share per GPU)
+    with how many distinct physical cores they are."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            kv[k.strip()] = v.strip()
+        tpc = int(kv.get("Thread(s) per core", "1") or 1)
+        cps = int(kv.get("Core(s) per socket", "0") or 0)
+        sk = int(kv.get("Socket(s)", "1") or 1)
+        info.update(model=kv.get("Model name"), sockets=sk, physical_cores=cps * sk,
+                    hardware_threads=int(kv.get("CPU(s)", "0") or 0), threads_per_core=tpc)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            info["smt_active"] = f.read().strip() == "1"
+    except OSError:
+        info["smt_active"] = info.get("threads_per_core", 1) > 1
+    cpus = sorted(os.sched_getaffinity(0))[:16]
+    cores = set()
+    for c in cpus:
+        try:
+            with open(f"/sys/devices/system/cpu/cpu{c}/topology/thread_siblings_list") as f:
+                cores.add(f.read().strip())
+        except OSError:
+            cores.add(str(c))
+    info["cpus_used"] = len(cpus)
+    info["physical_cores_used"] = len(cores)
+    info["smt_used"] = len(cores) < len(cpus)
+    return info, len(cpus)
+
+
 def cpu_baseline(ptm, ub, seconds: float):
-    """The reference's own bound / IVM code (oracle/_ref) single-threaded on a
-    bounded sample of the same workload; falls back to the C oracle port."""
+    """The reference's own bound / IVM code (oracle/_ref/ref_harness, one
+    single-threaded process per core) on the same instance and incumbent for
+    a bounded time: the search space [0, n!-1] is cut into 64 equal closed
+    factoradic intervals per core (as the reference's master cuts its work,
+    distributed/src/works.cpp:96-128, and hands intervals to
+    matrix_controller::initFromFac, multicore/src/matrix_controller.cpp:112-140);
+    a pool of one worker per core takes them in order, each explored from
+    its start until done or the deadline.  nodes/s = decomposed nodes of all
+    workers / wall.  Also a single-thread run from the root (the parity
+    configuration).  Falls back to the C oracle port where the reference
+    build is absent."""
+    import concurrent.futures as cf
     import oracle as O
+    from pbbperm_amd import factoradic as F
     O.lib()
-    if O.have_ref():
-        r = O.ref_solve(ptm, ub, bound_mode=0, branching=1, time_limit=seconds)
-        kind = "reference"
+    n = ptm.shape[1]
+    ref = O.have_ref()
+    kind = "reference" if ref else "port"
+    host, cores = host_cpus()
+    # one thread, from the root
+    if ref:
+        r1 = O.ref_solve(ptm, ub, bound_mode=0, branching=1, time_limit=seconds / 2)
     else:
-        t0 = time.time()
-        r = O.solve(ptm, ub, bound_mode=0, branching=1, max_nodes=int(seconds * 2e5))
-        r.wall_s = time.time() - t0
-        kind = "port"
-    return {"value": r.decomposed / r.wall_s, "unit": "nodes/s", "cores": 1, "kind": kind,
-            "sample": f"single-thread DFS from the root of the same instance/UB for {r.wall_s:.1f} s "
-                      f"({r.decomposed} decomposed nodes)"}
+        t = time.time()
+        r1 = O.solve(ptm, ub, bound_mode=0, branching=1, max_nodes=int(seconds / 2 * 2e5))
+        r1.wall_s = time.time() - t
+    single = {"value": r1.decomposed / r1.wall_s, "unit": "nodes/s", "cores": 1,
+              "sample": f"DFS from the root for {r1.wall_s:.1f} s ({r1.decomposed} decomposed nodes)"}
+    pos, end = F.split(n, 64 * cores)
+    t0 = time.time()
+    deadline = t0 + seconds
+
+    def run(k):
+        left = deadline - time.time()
+        if left <= 0.05:
+            return 0
+        if ref:
+            r = O.ref_solve(ptm, ub, bound_mode=0, branching=1, time_limit=left, pos=pos[k], end=end[k])
+        else:
+            r = O.solve(ptm, ub, bound_mode=0, branching=1, max_nodes=int(left * 2e5), pos=pos[k], end=end[k])
+        return r.decomposed
+
+    with cf.ThreadPoolExecutor(cores) as ex:
+        counts = list(ex.map(run, range(len(pos))))
+    wall = time.time() - t0
+    started = sum(1 for c in counts if c > 0)
+    return {"value": sum(counts) / wall, "unit": "nodes/s", "cores": cores, "kind": kind,
+            "sample": f"{cores} single-thread processes over {started} of {len(pos)} equal factoradic intervals "
+                      f"of the same instance and incumbent for {wall:.1f} s ({sum(counts)} decomposed nodes)",
+            "host": host, "single_thread": single}
 
 
 def note(msg: str):
     print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
 
-def _latency(lat, wall):
-    """Throughput and latency figures of a replay over the requests completed
-    within its time budget (the replay marks the others with latency -1;
-    they are counted in "late")."""
+def _latency(lat, wall, res=None):
+    """Throughput and latency figures of a replay.  A request is solved when
+    its result says so; the others are "late": still running when the
+    replay's time budget ended (latency -1), or stopped by the per-request
+    budget (status aborted, latency = when that result came back).  p50 /
+    p99 / mean are over the solved requests; p99_all also counts every late
+    request, at its latency or, when it never came back, at the replay's
+    end, so it is a lower bound of the stream's true p99."""
     import numpy as np
     a = np.asarray(lat, dtype=float)
-    done = a[a >= 0]
+    ok = a >= 0
+    if res is not None:
+        ok = ok & np.array([r.status == "solved" for r in res], dtype=bool)
+    done = a[ok]
     has = len(done) > 0
+    allv = np.where(a >= 0, a, max(float(a.max()) if len(a) else 0.0, wall * 1e3))
     return {"solves_per_s": len(done) / wall if wall > 0 else 0.0,
             "p50_ms": float(np.percentile(done, 50)) if has else None,
             "p99_ms": float(np.percentile(done, 99)) if has else None,
+            "p99_all_ms": float(np.percentile(allv, 99)) if len(allv) else None,
             "mean_ms": float(np.mean(done)) if has else None,
             "max_ms": float(np.max(done)) if has else None,
             "late": int(len(a) - len(done))}
@@ -205,12 +324,14 @@ def leg_batched(P, PP, args):
          f"margin={best.steal_margin}")
     with PP.Pool(cfg, params=best) as pool:
         res, _ = pool.solve(ptms, ubs, time_budget=120.0)
-        walls, kms = [], []
+        walls, kms, opss = [], [], []
         for _ in range(3):
             t = time.perf_counter()
             res, km = pool.solve(ptms, ubs, time_budget=120.0)
             walls.append(time.perf_counter() - t)
             kms.append(km)
+            hd, hr = pool.batch_histograms(len(ptms))
+            opss.append(pool_ops(ptms, hd, hr, lambda m: m * (m - 1) // 2))
         info = pool.info()
     note(f"batched: pool {min(walls) * 1e3:.1f} ms; plain path")
     # the plain path gets the same per-instance explorer count
@@ -227,7 +348,19 @@ def leg_batched(P, PP, args):
         ref_match = [r.best for r in res] == opts
     wall = min(walls)
     nodes = sum(r.decomposed for r in res)
+    # the roofline of k_pool<10,2>: the batch's algorithmic int32 ops over the
+    # graph's device time (HIP events on the pool's stream around the graph:
+    # the k_pool launch plus k_pool_init and the 2.6 MB request upload and
+    # 0.2 MB result download, which take well under 1 % of it)
+    km_avg = sum(kms) / len(kms)
+    ops_avg = sum(opss) / len(opss)
+    achieved = ops_avg / (km_avg / 1e3) / 1e12
     out = {
+        "roofline": {"bound": "valu", "kernel": "k_pool<10,2>", "achieved": achieved, "peak": PEAK_INT32_TOPS,
+                     "unit": "TOP/s (int32)", "frac": achieved / PEAK_INT32_TOPS,
+                     "ops_per_launch": ops_avg, "graph_ms_avg": km_avg,
+                     "traffic": BATCH_HBM_BYTES, "traffic_unit": "HBM bytes per launch (rocprofv3 "
+                     "FETCH_SIZE/WRITE_SIZE, calibrated)" if BATCH_HBM_BYTES else None},
         "workload": "512 instances, seeds 100-611, n~U[10,22], m~U{5,10}, LB1 + Johnson LB2 (all pairs), "
                     "MinMin branching, NEH incumbents, solved to optimality in one graph launch",
         "solves_per_s": len(ptms) / wall, "wall_ms": wall * 1e3, "graph_ms": min(kms),
@@ -248,78 +381,133 @@ def leg_batched(P, PP, args):
 
 
 def leg_serving(P, PP, args):
+    """The serving stream (n ~ U[12,30], m ~ U{5,10,20}, not folded) at a
+    fixed Poisson rate through the resident pool, through the two routed
+    pools, and through the plain one-instance-per-launch path, same arrivals
+    and the same per-request time budget everywhere: a request still running
+    that long after its admission is stopped and counted late."""
     import numpy as np
     N = args.serve_requests
     cfg = PP.workload_config("serving")
     ptms = PP.serving_stream(N, 10000)
     ubs = [P.neh(p)[0] for p in ptms]
     best, src = _params(PP, "serving", args)
+    best.request_budget_s = args.serve_budget
     pcfg = PP.workload_config("serving", nb_ivms=best.explorers_per_instance)
-    # arrival rate: a fraction of the plain path's closed-loop rate
-    k = min(N, 200)
-    _, _, w0 = PP.plain_replay(ptms[:k], ubs[:k], np.zeros(k), config=pcfg, time_budget=300.0)
-    rate = args.serve_load * k / w0
+    rate = args.serve_rate
     arrivals = PP.poisson_arrivals(N, rate, seed=7)
-    note(f"serving: {N} requests at {rate:.1f}/s through the pool")
+    span = float(arrivals[-1]) if N else 0.0
+    tb = span + 4 * args.serve_budget + 60.0
+    note(f"serving: {N} requests at {rate:.1f}/s ({span:.0f} s of arrivals), budget {args.serve_budget} s "
+         f"per request, through the pool")
     with PP.Pool(cfg, params=best) as pool:
-        res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=900.0)
+        res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=tb)
     note(f"serving: pool done in {wall:.1f} s; routed pools")
     rprm = PP.serving_routed_params(best)
     with PP.Pool(cfg, params=rprm[1]) as p20, PP.Pool(cfg, params=rprm[0]) as p10:
-        rres, rlat, rwall = PP.replay_routed([p20, p10], ptms, ubs, arrivals, time_budget=900.0)
+        rres, rlat, rwall = PP.replay_routed([p20, p10], ptms, ubs, arrivals, time_budget=tb)
     note(f"serving: routed done in {rwall:.1f} s; plain path")
-    plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=pcfg, time_budget=900.0)
+    plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=pcfg, time_budget=tb,
+                                         request_budget=args.serve_budget)
     note(f"serving: plain done in {pwall:.1f} s")
-    ok = all(r.status == "solved" for r in res + rres + plain)
-    same = all(a.best == b.best for a, b in zip(res, plain))
-    rsame = all(a.best == b.best for a, b in zip(rres, plain))
+
+    def agree(a, b):
+        return all(x.best == y.best for x, y in zip(a, b) if x.status == "solved" and y.status == "solved")
+
+    by_m = {}
+    for p, r, l in zip(ptms, res, lat):
+        d = by_m.setdefault(int(p.shape[0]), {"requests": 0, "late": 0})
+        d["requests"] += 1
+        d["late"] += int(r.status != "solved" or l < 0)
     return {
-        "workload": f"{N} Poisson requests, seeds 10000+, n~U[12,30], m~U{{5,10,20}} (m=20 folded to n<=16), "
-                    "LB1 + Johnson LB2, MinMin, NEH incumbents, every request solved to optimality",
-        "rate_per_s": rate, "load_vs_plain": args.serve_load,
-        **_latency(lat, wall),
+        "workload": f"{N} Poisson requests at {rate:g}/s, seeds 10000-{10000 + N - 1}, n~U[12,30], "
+                    "m~U{5,10,20}, LB1 + Johnson LB2 (all pairs), MinMin, NEH incumbents, per-request budget "
+                    f"{args.serve_budget:g} s after admission (over-budget requests are stopped and counted late)",
+        "requests": N, "rate_per_s": rate, "request_budget_s": args.serve_budget,
+        **_latency(lat, wall, res),
+        "late_by_machines": by_m,
         "decomposed": int(sum(r.decomposed for r in res)),
         "routed": {"pools": [{"machines": q.machines, "max_jobs": q.max_jobs, "groups": q.groups} for q in rprm],
-                   **_latency(rlat, rwall), "optima_equal_plain": rsame},
-        "plain": {**_latency(plat, pwall), "explorers_per_instance": best.explorers_per_instance},
-        "all_solved": ok, "optima_equal_plain": same,
+                   **_latency(rlat, rwall, rres), "optima_equal_plain": agree(rres, plain)},
+        "plain": {**_latency(plat, pwall, plain), "explorers_per_instance": best.explorers_per_instance},
+        "optima_equal_plain": agree(res, plain),
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
                   "steal_margin": best.steal_margin, "source": src},
     }
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--node-budget", type=int, default=400_000_000)
-    ap.add_argument("--explorers", type=int, default=1 << 20)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--legs", default="nodes,small,batched,serving")
-    ap.add_argument("--serve-requests", type=int, default=600)
-    ap.add_argument("--autotune", action="store_true", help="re-run the pool autotuner instead of tuned.json")
-    ap.add_argument("--serve-load", type=float, default=0.8,
-                    help="arrival rate as a fraction of the plain path's closed-loop solves/s")
-    ap.add_argument("--copy-probe", type=int, default=0,
-                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
+def leg_treesizes(P, PP, args):
+    """Tree sizes of the serving stream's distribution (its first
+    --tree-requests requests): every request solved alone in turn through a
+    pool with the serving tuning and a per-request budget of
+    --tree-budget s, NEH incumbent.  Reports decomposed nodes per request
+    (a lower bound for the requests the budget stopped) as a histogram by
+    machine count and decade."""
     import numpy as np
+    K = args.tree_requests
+    ptms = PP.serving_stream(K, 10000)
+    ubs = [P.neh(p)[0] for p in ptms]
+    best, _ = _params(PP, "serving", args)
+    best.request_budget_s = args.tree_budget
+    best.slots = 1
+    rows = []
+    with PP.Pool(PP.workload_config("serving"), params=best) as pool:
+        for i, (p, u) in enumerate(zip(ptms, ubs)):
+            t = time.perf_counter()
+            r, _ = pool.solve([p], [u], time_budget=args.tree_budget + 30.0)
+            rows.append((int(p.shape[0]), int(p.shape[1]), r[0].status, int(r[0].decomposed),
+                         time.perf_counter() - t))
+            if (i + 1) % 25 == 0:
+                note(f"tree sizes: {i + 1} / {K}")
+    hist = {}
+    for m, n, st, dec, sec in rows:
+        dcd = int(np.floor(np.log10(max(dec, 1))))
+        key = f"m{m}"
+        h = hist.setdefault(key, {})
+        k2 = f"1e{dcd}" + ("+" if st != "solved" else "")
+        h[k2] = h.get(k2, 0) + 1
+    return {"requests": K, "budget_s": args.tree_budget, "histogram_by_machines": hist,
+            "rows": [{"m": m, "n": n, "status": st, "decomposed": dec, "seconds": round(sec, 4)}
+                     for m, n, st, dec, sec in rows]}
+
+
+def leg_long(P, args):
+    """The headline workload run for --long-seconds of node-budgeted steps:
+    the rate of every step, to show whether the short timed window is
+    representative of a long (120 s-capped) run."""
+    n, m, seed = 50, 20, 2
+    ptm = P.synth(seed, n, m)
+    ub, _ = P.neh(ptm)
+    cfg = P.default_config(bound_mode=0, branching=1, refine=0, johnson_pairs=0, early_stop_johnson=0,
+                           nb_ivms=args.explorers)
+    curve = []
+    with P.GpuBB(ptm, cfg, ub=ub) as eng:
+        eng.init_full_interval()
+        t0 = time.perf_counter()
+        prev = eng.stats().decomposed
+        while time.perf_counter() - t0 < args.long_seconds:
+            t = time.perf_counter()
+            done = eng.next(node_budget=args.node_budget)
+            st = eng.stats()
+            curve.append((round(time.perf_counter() - t0, 2), (st.decomposed - prev) / (time.perf_counter() - t)))
+            prev = st.decomposed
+            if done:
+                break
+        total = time.perf_counter() - t0
+        st = eng.stats()
+        best, _ = eng.best()
+    rates = [r for _, r in curve]
+    return {"seconds": total, "steps": len(curve), "decomposed": st.decomposed, "nodes_per_s": st.decomposed / total,
+            "step_rate_min": min(rates), "step_rate_max": max(rates),
+            "step_rate_first": rates[0], "step_rate_last": rates[-1], "best": int(best),
+            "curve": [[t, round(r)] for t, r in curve]}
+
+
+def leg_nodes(P, args):
+    """The headline: decomposed nodes/s on the 50x20 instance, K timed
+    node-budgeted launches after W warm-up launches, with the int32 VALU
+    roofline of k_explore<20,0> and the reference CPU baseline."""
     import torch
-    import pbbperm_amd as P
-
-    if args.copy_probe:
-        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-
     n, m, seed = 50, 20, 2
     ptm = P.synth(seed, n, m)
     ub, _ = P.neh(ptm)
@@ -364,8 +552,12 @@ def main():
         "data": "synthetic (SplitMix64 seed 2, times uniform [1,99])",
         "config": {"workload": "one 50x20 PFSP instance, LB1 bounding, MaxSum branching, NEH incumbent, "
                                "node-budgeted resident launches",
-                   "n": n, "m": m, "seed": seed, "explorers": st1.explorers, "ub": int(ub), "best": int(best),
-                   "node_budget_per_step": args.node_budget},
+                   "n": n, "m": m, "seed": seed, "explorers": st1.explorers,
+                   "explorers_stated": "16384-65536",
+                   "explorers_note": "one explorer per 64-lane wavefront; 4096 is the resident maximum of "
+                                     "k_explore<20,0> on gfx950 (128 VGPRs: 4 waves per SIMD x 4 SIMDs x 256 CUs), "
+                                     "and a persistent launch needs every explorer resident",
+                   "ub": int(ub), "best": int(best), "node_budget_per_step": args.node_budget},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_INT32_TOPS, "unit": "TOP/s (int32)",
                      "frac": achieved / PEAK_INT32_TOPS, "traffic": traffic,
                      "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
@@ -375,18 +567,63 @@ def main():
                      "kernel_ms_per_step": kernel_ms / args.steps,
                      "ops_per_step": ops / args.steps},
     }
+    note(f"nodes leg: {value / 1e6:.1f} M nodes/s, roofline frac {achieved / PEAK_INT32_TOPS:.3f}")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(ptm, ub, args.cpu_seconds)
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--node-budget", type=int, default=400_000_000)
+    ap.add_argument("--explorers", type=int, default=1 << 20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="nodes,small,batched,serving",
+                    help="any of nodes,small,batched,serving (default) and treesizes,long")
+    ap.add_argument("--serve-requests", type=int, default=SERVE_REQUESTS)
+    ap.add_argument("--serve-rate", type=float, default=SERVE_RATE, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-budget", type=float, default=SERVE_BUDGET_S,
+                    help="per-request time budget after admission (s)")
+    ap.add_argument("--autotune", action="store_true", help="re-run the pool autotuner instead of tuned.json")
+    ap.add_argument("--tree-requests", type=int, default=300)
+    ap.add_argument("--tree-budget", type=float, default=20.0)
+    ap.add_argument("--long-seconds", type=float, default=120.0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of this build",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    import pbbperm_amd as P
+
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * (1 << 20) // 4, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+
     legs = set(args.legs.split(","))
+    line = leg_nodes(P, args) if "nodes" in legs else {"legs": sorted(legs)}
     if "small" in legs:
         line["small_configs"] = leg_small(P, args)
         note(f"small configs: {line['small_configs']}")
-    note(f"nodes leg: {value / 1e6:.1f} M nodes/s")
-    if legs & {"batched", "serving"}:
+    if "long" in legs:
+        line["long"] = leg_long(P, args)
+        note(f"long run: {line['long']['nodes_per_s'] / 1e6:.1f} M nodes/s over {line['long']['seconds']:.0f} s")
+    if legs & {"batched", "serving", "treesizes"}:
         import pbbperm_amd.pool as PP
         if "batched" in legs:
             line["batched"] = leg_batched(P, PP, args)
             print(json.dumps({"leg": "batched", **line["batched"]}), file=sys.stderr, flush=True)
+        if "treesizes" in legs:
+            line["treesizes"] = leg_treesizes(P, PP, args)
         if "serving" in legs:
             line["serving"] = leg_serving(P, PP, args)
     print(json.dumps(line), flush=True)

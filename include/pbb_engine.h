@@ -47,7 +47,9 @@ typedef struct pbb_config {
     int refine;             /* refinement bound of modes 1/2: 0 = LB1' as shipped
                                (ivm_bound.cpp:170), 1 = Johnson LB2 (ivm_bound.cpp:171) */
     int johnson_pairs;      /* [bb] JohnsonMode: 0 all pairs, 1 (i,m), 2 (i,i+1) */
-    int early_stop_johnson; /* [bb] earlyStopJohnson (only prune-equivalent uses) */
+    int early_stop_johnson; /* [bb] earlyStopJohnson: accepted where it cannot change
+                               the search (boundingMode 0, 2, or 1 with refine 0);
+                               PBB_E_ARG with boundingMode 1 and refine 1 */
     int find_all;           /* [bb] findAll: prune with > instead of >= */
     int single_node;        /* [bb] singleNode: MinBranch/MinMin use the live UB (1)
                                or the UB at engine creation (0) */
@@ -68,8 +70,8 @@ int pbb_config_default(pbb_config *cfg);
 /* Parse a gpuconfig.ini/bbconfig.ini file (same sections and keys as the
  * reference).  problem/instance receive [problem] problem/instance (buffers of
  * at least 64 bytes, may be NULL).  Unsupported values (sortedDFS != 0,
- * JohnsonMode 3, branching -2/-3 with bound modes that read back bounds of
- * the reference's stale arrays) return PBB_E_ARG. */
+ * JohnsonMode 3, earlyStopJohnson with boundingMode 1 and the Johnson
+ * refinement, a negative steal margin) return PBB_E_ARG. */
 int pbb_config_load_ini(const char *path, pbb_config *cfg, char *problem, char *instance);
 
 /* ------------------------------------------------------------- instances */
@@ -140,11 +142,16 @@ typedef struct pbb_stats {
     int active;              /* explorers holding an interval after the last run */
     double kernel_ms;        /* device time of the last pbb_run */
     uint64_t depth_histo[PBB_MAX_JOBS + 2]; /* decomposed nodes per depth */
+    uint64_t refine_histo[PBB_MAX_JOBS + 2]; /* refinement bounds per depth of the bounded node */
 } pbb_stats;
 
 /* gpubb(pbab*) + initialize + initializeBoundFSP + copyH2D
  * (gpu/src/gpubb.cu:24-91, :974-1038).  ub is the incumbent cost (INT_MAX
- * for none), ub_perm its permutation (may be NULL). */
+ * for none; negative is PBB_E_ARG), ub_perm its permutation (may be NULL;
+ * PBB_E_ARG when not a permutation of 0..n-1).  The engine sizes its grid
+ * for the whole device; explorers only receive work once their wavefront
+ * runs, so another kernel on the device delays the search but cannot stall
+ * it. */
 int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cfg, int ub,
                const int *ub_perm, pbb_engine **out);
 int pbb_destroy(pbb_engine *e);
@@ -263,6 +270,9 @@ typedef struct pbb_pool_params {
     int steal_margin;           /* split only at lines < n - margin; 0 = the config's */
     int groups;                 /* workgroups launched; 0 = every resident one */
     double hbm_budget_gb;       /* serving admission budget; 0 = device memory free at creation */
+    double request_budget_s;    /* per-instance time after admission (0 = none): an instance
+                                   still running then is stopped and retires PBB_RES_ABORTED,
+                                   with the best schedule found so far */
 } pbb_pool_params;
 
 typedef struct pbb_result {
@@ -284,6 +294,14 @@ int pbb_pool_info(pbb_pool *p, int *explorers, int *groups, int *lds_bytes);
 /* Change the per-instance explorer cap and the steal margin of an idle pool
  * (the cap stays within the idle-ring capacity chosen at creation). */
 int pbb_pool_retune(pbb_pool *p, int explorers_per_instance, int steal_margin);
+
+/* Depth histograms of the last pbb_pool_solve batch, instance i at
+ * [64 i .. 64 i + 63]: decomposed[.. + d] = nodes decomposed with d jobs
+ * fixed (leaves included, at d = n - 1), refined[.. + d] = refinement bounds
+ * (LB1' or Johnson) computed for the children of such nodes.  For roofline
+ * accounting: the work of a node and of a refinement depends on d.  count
+ * must not exceed the last batch's count; PBB_E_STATE while serving. */
+int pbb_pool_batch_histograms(pbb_pool *p, int count, uint64_t *decomposed, uint64_t *refined);
 
 /* A request's ub may be PBB_UB_NEH: the admitting workgroup then computes
  * the incumbent with NEH on the device (pbb_neh_device semantics) and the
@@ -342,10 +360,14 @@ int pbb_serve_replay_routed(pbb_pool *const *pools, int npools, int count, const
                             const int *m, const int *ub, const double *arrival_s, double time_budget_s,
                             pbb_result *out, double *latency_ms, double *wall_s);
 /* The same stream on the one-instance engine, one launch per instance in
- * arrival order (first come, first served). */
+ * arrival order (first come, first served).  ub[i] may be PBB_UB_NEH (the
+ * host NEH's incumbent, computed after the request arrives).
+ * request_budget_s (0 = none) caps each instance's launch as the pool's
+ * request_budget_s does: an instance still running then retires
+ * PBB_RES_ABORTED with the best schedule found so far. */
 int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, const int *n,
                      const int *m, const int *ub, const double *arrival_s, double time_budget_s,
-                     pbb_result *out, double *latency_ms, double *wall_s);
+                     double request_budget_s, pbb_result *out, double *latency_ms, double *wall_s);
 
 /* Autotuner: time pbb_pool_solve on the given instances for every
  * (waves_per_group in {1,2,4}, explorers_per_instance in {64,256,1024},

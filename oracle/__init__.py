@@ -9,9 +9,11 @@ unmodified).  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
 from __future__ import annotations
 
 import ctypes as C
+import itertools
 import json
 import os
 import subprocess
+import threading
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
@@ -298,9 +300,13 @@ def have_ref() -> bool:
     return os.path.exists(REF_HARNESS)
 
 
+_tmp_seq = itertools.count()
+
+
 def _write_tmp(ptm: np.ndarray, tmpdir: str) -> str:
+    """A file of its own per call (callers run the harness from many threads)."""
     os.makedirs(tmpdir, exist_ok=True)
-    path = os.path.join(tmpdir, f"ptm_{os.getpid()}_{abs(hash(ptm.tobytes()))}.txt")
+    path = os.path.join(tmpdir, f"ptm_{os.getpid()}_{threading.get_ident()}_{next(_tmp_seq)}.txt")
     with open(path, "w") as f:
         f.write(ptm_text(ptm))
     return path

@@ -47,6 +47,7 @@ class Stats(C.Structure):
         ("decomposed", C.c_uint64), ("leaves", C.c_uint64), ("simple_bounds", C.c_uint64),
         ("johnson_bounds", C.c_uint64), ("explorers", C.c_int), ("active", C.c_int),
         ("kernel_ms", C.c_double), ("depth_histo", C.c_uint64 * (PBB_MAX_JOBS + 2)),
+        ("refine_histo", C.c_uint64 * (PBB_MAX_JOBS + 2)),
     ]
 
 
@@ -57,6 +58,7 @@ class PoolParams(C.Structure):
         ("max_jobs", C.c_int), ("machines", C.c_int), ("slots", C.c_int),
         ("waves_per_group", C.c_int), ("explorers_per_instance", C.c_int),
         ("steal_margin", C.c_int), ("groups", C.c_int), ("hbm_budget_gb", C.c_double),
+        ("request_budget_s", C.c_double),
     ]
 
     def as_dict(self):
@@ -137,8 +139,9 @@ SIGNATURES = {
                                           C.POINTER(C.c_double), C.c_double, C.POINTER(Result),
                                           C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pbb_plain_replay": (C.c_int, [C.POINTER(Config), C.c_int, I32P, IP, IP, IP, C.POINTER(C.c_double),
-                                   C.c_double, C.POINTER(Result), C.POINTER(C.c_double),
+                                   C.c_double, C.c_double, C.POINTER(Result), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]),
+    "pbb_pool_batch_histograms": (C.c_int, [VP, C.c_int, U64P, U64P]),
     "pbb_pool_autotune": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.c_int, I32P, IP, IP, IP,
                                     C.POINTER(PoolParams), C.POINTER(C.c_double), C.c_int, IP]),
     "pbb_fact_to_dec": (C.c_int, [C.c_int, IP, C.c_char_p, C.c_int]),

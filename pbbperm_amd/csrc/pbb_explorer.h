@@ -278,7 +278,8 @@ struct Explorer {
     unsigned dec, leaves, sb, jb;     // deltas since the last flush
     unsigned nchk;                    // periodic checks so far (host-word phase)
     bool end_dirty;                   // end moved since pos was last compared with it
-    int hist;
+    int hist;                         // lane d: nodes decomposed at depth d since the flush
+    int jhist;                        // lane d: refinement bounds at depth d (REF != 0)
     unsigned long long t_end;
 
     __device__ __forceinline__ Explorer(const KArgs &args, const Lds &l, int eid, const InstRef &r, Ctl *runctl,
@@ -293,7 +294,7 @@ struct Explorer {
         dec = leaves = sb = jb = 0;
         nchk = (unsigned)eid;  // staggers the host-word reads across explorers
         end_dirty = true;
-        hist = 0;
+        hist = jhist = 0;
         local_best = PBB_INF;
         t_end = args.deadline ? wall_clock64() + args.deadline : 0ull;
     }
@@ -631,6 +632,7 @@ struct Explorer {
         bool todo = valid && !gt_best(cw);
         uint64_t tmask = ballot(todo);
         jb += (unsigned)__popcll(tmask);
+        jhist += lane == L ? __popcll(tmask) : 0;
         if constexpr (REF == 1) {
             int chF[M], chB[M];
             child_chains(job, chF, chB);
@@ -654,6 +656,7 @@ struct Explorer {
     {
         const int k = n - L;
         jb += 2u * (unsigned)k;
+        jhist += lane == L ? 2 * k : 0;
         if constexpr (REF == 1) {
             int chF[M], chB[M];
             child_chains(job, chF, chB);
@@ -901,7 +904,8 @@ struct Explorer {
 
     // bbthread::shareWork + ivm::cuttingPosition (bbthread.cpp:39-72,
     // ivm.cpp:188-237): split at the first line l whose pos differs from end
-    // (l <= L, l < n - margin); the thief gets the explorable cells after the
+    // (l <= L, l < n - margin, the size_d - margin rule of
+    // gpu/include/loadbalance.cuh:90); the thief gets the explorable cells after the
     // victim's kept half, rows 0..l and the victim's end; no re-bounding.
     // A line whose cells after pos are all pruned or visited holds no work:
     // its end is trimmed to pos (deeper ends become the maximum, so the
@@ -914,8 +918,8 @@ struct Explorer {
         h = (unsigned)uni((int)h);
         t = (unsigned)uni((int)t);
         if (h == t) return false;
-        int lim = min(L, n - a.steal_margin);
-        if (lim < 0) lim = 0;
+        const int lim = min(L, n - 1 - a.steal_margin);  // last line a split may use
+        if (lim < 0) return true;                         // margin >= n: never split
         uint64_t ne = ballot(lane < n && posv != endv);
         uint64_t limmask = lim >= 64 ? ~0ull : ((1ull << lim) - 1ull);
         uint64_t cand = ne & limmask;
@@ -1037,6 +1041,10 @@ struct Explorer {
         dec = leaves = sb = jb = 0;
         if (hist && lane <= n) atomicAdd(&ir.ctl->histo[lane], (unsigned long long)hist);
         hist = 0;
+        if constexpr (REF != 0) {
+            if (jhist && lane <= n) atomicAdd(&ir.ctl->jhisto[lane], (unsigned long long)jhist);
+            jhist = 0;
+        }
     }
 
     // incumbent word loaded by the caller (cost << 32 | explorer)
@@ -1056,6 +1064,25 @@ struct Explorer {
         }
         if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
         if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(&run->stop, 1);
+        poll_host_and_best();
+        return uni(ld_rlx(&run->stop)) != 0;
+    }
+
+    // The pool's periodic check: the deadline (the launch's or the request's,
+    // whichever is first) raises the instance's own stop word.  Returns 0 to
+    // go on, 1 when the launch stops (interrupt, error: the explorer leaves),
+    // 2 when only this instance stops.
+    __device__ __forceinline__ int check_stop_slot(int *slot_stop)
+    {
+        if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
+        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(slot_stop, 1);
+        poll_host_and_best();
+        if (uni(ld_rlx(&run->stop)) != 0) return 1;
+        return uni(ld_rlx(slot_stop)) != 0 ? 2 : 0;
+    }
+
+    __device__ __forceinline__ void poll_host_and_best()
+    {
         const bool host_turn = (nchk++ & (PBB_HOST_POLL_CHECKS - 1u)) == 0u;
         if (a.host_flag && host_turn && lane == 0) {
             int hf = __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1064,7 +1091,6 @@ struct Explorer {
         unsigned long long b = ld_rlx(ir.best);
         int bc = uni((int)(unsigned)(b >> 32));
         if (bc < local_best) local_best = bc;
-        return uni(ld_rlx(&run->stop)) != 0;
     }
 };
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cctype>
 #include <chrono>
+#include <cstddef>
 #include <climits>
 #include <cstdarg>
 #include <cstdio>
@@ -76,6 +77,21 @@ int pbb::validate(const pbb_config *c)
         return fail(PBB_E_ARG, "sortedDFS=%d unsupported: the engine explores siblings in IVM row order (sortedDFS=0)",
                     c->sorted_dfs);
     if (c->nb_ivms < 1) return fail(PBB_E_ARG, "nbIVMs must be >= 1");
+    if (c->steal_margin < 0) return fail(PBB_E_ARG, "steal margin %d is negative", c->steal_margin);
+    // earlyStopJohnson (bound_fsp_strong.cpp:114-123, :150-154): the Johnson
+    // bound stops at the first pair whose value exceeds the incumbent and
+    // reorders its pairs by past success, state of each bound object.  With
+    // boundingMode 2 the refined value only prunes, and a value above the
+    // incumbent prunes whether or not the loop stopped early, so the search
+    // is unchanged and the option is accepted.  With boundingMode 1 the
+    // Johnson values also choose the branching direction
+    // (ivm_bound.cpp:668-676), so the early values would steer the search;
+    // the engine computes every pair, and refuses the combination rather than
+    // return different counts silently.
+    if (c->early_stop_johnson && c->bound_mode == 1 && c->refine == 1)
+        return fail(PBB_E_ARG,
+                    "earlyStopJohnson=true with boundingMode 1 and the Johnson refinement is unsupported "
+                    "(its early values choose the branching direction); set earlyStopJohnson=false");
     return PBB_OK;
 }
 
@@ -571,6 +587,10 @@ struct pbb_engine {
     bool root_bounded = false;
     bool initialized = false;
     unsigned long long exported_dec = 0, exported_leaves = 0;
+    // counters carried in from a checkpoint file (works::init_infile restores
+    // simpleBounds, distributed/src/works.cpp:81-83; the unit's exploredNodes)
+    unsigned long long dec_base = 0, sb_base = 0;
+    std::vector<unsigned> ring_seq0;  // the empty ring's cell sequence numbers
     double last_ms = 0.0;
     int wall_khz = 100000;
 };
@@ -657,6 +677,10 @@ extern "C" int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cf
     int rc = check_shape(ptm, n, m);
     if (rc) return rc;
     if ((rc = validate(cfg))) return rc;
+    // an incumbent cost is a makespan: a negative one (PBB_UB_NEH of the pool
+    // entry points, say) would prune every root child and report it as optimal
+    if (ub < 0) return fail(PBB_E_ARG, "incumbent cost %d is negative (INT_MAX = none)", ub);
+    if (ub_perm && !is_perm(ub_perm, n)) return fail(PBB_E_ARG, "incumbent order is not a permutation of 0..%d", n - 1);
     if ((rc = check_device())) return rc;
     pbb_engine *e = new pbb_engine();
     e->cfg = *cfg;
@@ -777,30 +801,36 @@ static int setup_pool(pbb_engine *e, const std::vector<int> &ids, const int *pos
             he[(size_t)id * 64 + i] = (int8_t)end[k * n + i];
         }
     }
-    std::vector<int> ring_id(e->ring_cap, 0);
-    std::vector<unsigned> ring_seq(e->ring_cap);
-    for (int i = 0; i < e->ring_cap; i++) ring_seq[i] = (unsigned)i;
-    unsigned tail = 0;
-    for (int i = 0; i < E; i++)
-        if (state[i] == ST_IDLE) {
-            ring_id[tail] = i + 1;
-            ring_seq[tail] = tail + 1;
-            tail++;
-        }
+    // The idle ring starts empty: idle explorers register themselves when
+    // their wavefront runs (pbb_run empties it again before every launch).
     Ctl c;
     memset(&c, 0, sizeof c);
-    c.idle_head = 0;
-    c.idle_tail = tail;
     c.work_count = (int)ids.size();
     c.done = ids.empty() ? 1 : 0;
     HIPCHK(hipMemcpy(e->state, state.data(), (size_t)E * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->pos, hp.data(), hp.size(), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->end, he.data(), he.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->ring_id, ring_id.data(), (size_t)e->ring_cap * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->ring_seq, ring_seq.data(), (size_t)e->ring_cap * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(e->ctl, &c, sizeof c, hipMemcpyHostToDevice));
     e->exported_dec = e->exported_leaves = 0;
+    e->dec_base = e->sb_base = 0;
     e->initialized = true;
+    return PBB_OK;
+}
+
+// Empty the idle ring before a launch: cell i waits for ticket i, both ticket
+// counters at 0.  Explorers idle at the end of the previous launch register
+// again when they run, so the ring never names a wavefront that is not
+// resident.
+static int reset_ring(pbb_engine *e)
+{
+    if (e->ring_seq0.empty()) {
+        e->ring_seq0.resize(e->ring_cap);
+        for (int i = 0; i < e->ring_cap; i++) e->ring_seq0[i] = (unsigned)i;
+    }
+    HIPCHK(hipMemcpy(e->ring_seq, e->ring_seq0.data(), (size_t)e->ring_cap * 4, hipMemcpyHostToDevice));
+    const unsigned zero[2] = {0u, 0u};
+    static_assert(offsetof(Ctl, idle_tail) == offsetof(Ctl, idle_head) + sizeof(unsigned), "ring tickets");
+    HIPCHK(hipMemcpy(&e->ctl->idle_head, zero, sizeof zero, hipMemcpyHostToDevice));
     return PBB_OK;
 }
 
@@ -847,6 +877,8 @@ extern "C" int pbb_run(pbb_engine *e, uint64_t node_budget, double time_budget_s
     }
     c.stop = 0;
     HIPCHK(hipMemcpy(&e->ctl->stop, &c.stop, sizeof(int), hipMemcpyHostToDevice));
+    int rc0 = reset_ring(e);
+    if (rc0) return rc0;
     *e->host_flag = 0;
     KArgs a = make_args(e);
     a.node_budget = node_budget ? c.dec + node_budget : 0;
@@ -920,6 +952,8 @@ extern "C" int pbb_get_incumbent(pbb_engine *e, int *cost, int *perm)
 extern "C" int pbb_set_incumbent(pbb_engine *e, int cost, const int *perm)
 {
     if (!e) return fail(PBB_E_ARG, "null engine");
+    if (cost < 0) return fail(PBB_E_ARG, "incumbent cost %d is negative", cost);
+    if (perm && !is_perm(perm, e->n)) return fail(PBB_E_ARG, "incumbent order is not a permutation of 0..%d", e->n - 1);
     int rc = sync_incumbent(e);
     if (rc) return rc;
     if (cost >= e->inc_cost) return PBB_OK;  // solution::update keeps strict improvements
@@ -936,9 +970,9 @@ extern "C" int pbb_get_stats(pbb_engine *e, pbb_stats *st)
     Ctl c;
     HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
     memset(st, 0, sizeof *st);
-    st->decomposed = c.dec;
+    st->decomposed = e->dec_base + c.dec;
     st->leaves = c.leaves;
-    st->simple_bounds = c.sb;
+    st->simple_bounds = e->sb_base + c.sb;
     st->johnson_bounds = c.jb;
     st->explorers = e->E;
     std::vector<int> s(e->E);
@@ -947,7 +981,10 @@ extern "C" int pbb_get_stats(pbb_engine *e, pbb_stats *st)
     for (int v : s) act += v != ST_IDLE;
     st->active = act;
     st->kernel_ms = e->last_ms;
-    for (int i = 0; i <= PBB_MAX_JOBS + 1; i++) st->depth_histo[i] = c.histo[i];
+    for (int i = 0; i <= PBB_MAX_JOBS + 1; i++) {
+        st->depth_histo[i] = c.histo[i];
+        st->refine_histo[i] = c.jhisto[i];
+    }
     return PBB_OK;
 }
 
@@ -1016,7 +1053,7 @@ extern "C" int pbb_query_explorers(int n, int m, const pbb_config *cfg, int *exp
 // ------------------------------------------------- plain stream replay
 // The one-instance engine re-pointed at a new instance of the same shape:
 // tables re-uploaded into the existing buffers, incumbent and root reset.
-static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub)
+static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub, const int *ub_perm)
 {
     build_tables(ptm, e->n, e->m, e->T);
     const HostTables &T = e->T;
@@ -1030,7 +1067,10 @@ static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub)
     }
     e->inc_cost = ub;
     e->initial_ub = ub;
-    for (int i = 0; i < e->n; i++) e->inc_perm[i] = e->root_perm[i] = i;
+    for (int i = 0; i < e->n; i++) {
+        e->root_perm[i] = i;
+        e->inc_perm[i] = ub_perm ? ub_perm[i] : i;
+    }
     e->root_bounded = false;
     e->initialized = false;
     *e->host_flag = 0;
@@ -1041,10 +1081,11 @@ static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub)
 
 extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t *ptms, const int *n,
                                 const int *m, const int *ub, const double *arrival_s, double time_budget_s,
-                                pbb_result *out, double *latency_ms, double *wall_s)
+                                double request_budget_s, pbb_result *out, double *latency_ms, double *wall_s)
 {
     if (!cfg || count < 0 || (count > 0 && (!ptms || !n || !m || !ub || !arrival_s || !out || !latency_ms)))
         return fail(PBB_E_ARG, "bad replay arguments");
+    if (request_budget_s < 0) return fail(PBB_E_ARG, "negative request budget");
     std::vector<size_t> off(count + 1, 0);
     for (int i = 0; i < count; i++) {
         int rc = check_shape(ptms + off[i], n[i], m[i]);
@@ -1060,7 +1101,7 @@ extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t 
         auto key = std::make_pair(n[i], m[i]);
         if (eng.count(key)) continue;
         pbb_engine *e = nullptr;
-        int rc = pbb_create(ptms + off[i], n[i], m[i], cfg, ub[i], nullptr, &e);
+        int rc = pbb_create(ptms + off[i], n[i], m[i], cfg, INT_MAX, nullptr, &e);
         if (rc) {
             cleanup();
             return rc;
@@ -1081,10 +1122,21 @@ extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t 
     }
     for (int i = 0; i < count && rc == PBB_OK; i++) {
         while (now_s() < arrival_s[i]) std::this_thread::yield();
-        const double left = budget - now_s();
+        double left = budget - now_s();
         if (left <= 0) break;
+        if (request_budget_s > 0 && request_budget_s < left) left = request_budget_s;
         pbb_engine *e = eng[std::make_pair(n[i], m[i])];
-        if ((rc = engine_retarget(e, ptms + off[i], ub[i]))) break;
+        // PBB_UB_NEH (any negative ub): the incumbent is the host NEH's, as the
+        // pool computes it on the device at admission; its time counts in the
+        // request's latency there too
+        int u = ub[i];
+        std::vector<int> neh_perm;
+        if (u < 0) {
+            neh_perm.resize(n[i]);
+            for (int j = 0; j < n[i]; j++) neh_perm[j] = j;
+            if ((rc = pbb_neh(ptms + off[i], n[i], m[i], neh_perm.data(), &u))) break;
+        }
+        if ((rc = engine_retarget(e, ptms + off[i], u, u == ub[i] ? nullptr : neh_perm.data()))) break;
         const double t_admit = now_s();
         int all_done = 0;
         if ((rc = pbb_init_full(e))) break;
@@ -1098,7 +1150,8 @@ extern "C" int pbb_plain_replay(const pbb_config *cfg, int count, const int32_t 
         pbb_result &o = out[i];
         o.status = all_done ? PBB_RES_SOLVED : PBB_RES_ABORTED;
         o.best = cost;
-        o.improved = cost < ub[i] ? 1 : 0;
+        // improved: perm holds a schedule of cost best (the NEH order counts)
+        o.improved = (cost < u || u != ub[i]) ? 1 : 0;
         o.decomposed = st.decomposed;
         o.leaves = st.leaves;
         o.simple_bounds = st.simple_bounds;
@@ -1167,10 +1220,16 @@ extern "C" int pbb_checkpoint_load(pbb_engine *e, const char *path, pbb_checkpoi
     pbb_checkpoint c;
     int rc = pbb_checkpoint_parse(text.c_str(), n, &c, ids.data(), pos.data(), end.data(), E);
     if (rc) return rc;
+    if (!is_perm(c.best_perm, n)) return fail(PBB_E_ARG, "%s: incumbent is not a permutation of 0..%d", path, n - 1);
     if ((rc = pbb_set_root(e, c.root_perm))) return rc;
     if ((rc = pbb_set_incumbent(e, c.best_cost, c.best_perm))) return rc;
     for (int k = 0; k < c.nb; k++) ids[k] = k;  // interval k -> explorer k
     if ((rc = pbb_init_intervals(e, c.nb, ids.data(), pos.data(), end.data()))) return rc;
+    // the file's counters carry on (works::init_infile restores simpleBounds,
+    // distributed/src/works.cpp:81-83): pbb_get_stats and the next save
+    // report the whole run, not the segment since this load
+    e->dec_base = c.explored;
+    e->sb_base = c.simple_bounds;
     if (info) *info = c;
     return PBB_OK;
 }

@@ -66,7 +66,8 @@ struct Ctl {
     unsigned long long leaves;
     unsigned long long sb;     // simple (LB1) bounds
     unsigned long long jb;     // refinement bounds
-    unsigned long long histo[65];
+    unsigned long long histo[65];   // decomposed nodes per depth (fixed jobs)
+    unsigned long long jhisto[65];  // refinement bounds per depth of the bounded node
 };
 
 // The instance one explorer works on: its control block, idle ring,
@@ -266,6 +267,7 @@ struct PoolArgs {
     PoolQueue *q;
     unsigned *q_head;             // consumer ticket (device memory)
     Ctl *run;                     // stop / error of the launch
+    unsigned long long req_ticks; // per-request budget after admission (wall_clock64 ticks), 0 = none
 };
 
 __device__ __host__ inline size_t pool_lds_bytes(int nmax, int M, int pmax, bool jt_lds, int W)

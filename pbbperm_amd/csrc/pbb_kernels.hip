@@ -55,13 +55,23 @@ __global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_
         }
     } else if (st == ST_ACTIVE) {
         X.load_state();
+    } else {
+        // An idle explorer enters the idle ring only once it runs (the host
+        // leaves the ring empty at every launch), so donors only ever hand
+        // work to wavefronts that are resident: one still waiting for a CU
+        // (another kernel or process on the device, an occupancy estimate
+        // that was too high) holds no work that the others wait for.
+        X.register_idle();
     }
 
     unsigned iter = 0;
     bool hungry = true;
     while (true) {
         if (st == ST_IDLE) {
-            // wait for a donation, global termination or a stop
+            // wait for a donation, global termination or a stop; the host
+            // interrupt word (a PCIe round trip) is read by every 16th
+            // explorer, at every 16th pass once its back-off is at the
+            // longest, and the first to see it raises the device stop word
             int nap = 1;
             while (true) {
                 // relaxed poll (an acquire would invalidate the L2 at every
@@ -81,6 +91,9 @@ __global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_
                     X.flush_counters();
                     return;
                 }
+                if (a.host_flag && (e & 15) == 0 && nap == 16 && X.lane == 0 &&
+                    __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
+                    st_rlx(&a.ctl->stop, 1);
                 X.idle_pause(nap);
             }
         }

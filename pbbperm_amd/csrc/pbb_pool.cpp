@@ -68,7 +68,8 @@ struct pbb_pool {
     // batch graph
     hipGraphExec_t gexec = nullptr;
     int g_count = -1, g_epi = -1, g_margin = -1;
-    unsigned long long g_deadline = 0;
+    unsigned long long g_deadline = 0, g_req = 0;
+    int last_count = 0;  // instances of the last pbb_pool_solve batch
     // serving state
     bool serving = false;
     std::vector<int> free_slots;
@@ -131,6 +132,8 @@ static PoolArgs pool_pargs(const pbb_pool *p, bool serve)
     q.q = serve ? p->sq_d : p->q_d;
     q.q_head = p->q_head;
     q.run = p->run;
+    q.req_ticks = p->prm.request_budget_s > 0 ? (unsigned long long)(p->prm.request_budget_s * p->wall_khz * 1000.0)
+                                              : 0ull;
     return q;
 }
 
@@ -145,6 +148,7 @@ extern "C" int pbb_pool_params_default(pbb_pool_params *prm)
     prm->steal_margin = 0;
     prm->groups = 0;
     prm->hbm_budget_gb = 0.0;
+    prm->request_budget_s = 0.0;
     return PBB_OK;
 }
 
@@ -199,7 +203,7 @@ extern "C" int pbb_pool_create(const pbb_config *cfg, const pbb_pool_params *prm
         return fail(PBB_E_ARG, "waves_per_group must be 1, 2 or 4");
     if (prm->explorers_per_instance < prm->waves_per_group)
         return fail(PBB_E_ARG, "explorers_per_instance below waves_per_group");
-    if (prm->steal_margin < 0 || prm->groups < 0 || prm->hbm_budget_gb < 0)
+    if (prm->steal_margin < 0 || prm->groups < 0 || prm->hbm_budget_gb < 0 || prm->request_budget_s < 0)
         return fail(PBB_E_ARG, "negative pool parameter");
     if ((rc = check_device())) return rc;
 
@@ -367,9 +371,12 @@ extern "C" int pbb_pool_solve(pbb_pool *p, int count, const int32_t *ptms, const
     if (p->serving) return fail(PBB_E_STATE, "pool is serving");
     if (count > p->prm.slots) return fail(PBB_E_CAPACITY, "%d instances > %d slots", count, p->prm.slots);
     if (count == 0) return PBB_OK;
+    // The device record's tag is the request's index, which the kernel stores
+    // as the last word of a retired result (k_pool_init resets it to -1), so
+    // any caller tag -- -1 included -- leaves the retirement test exact.
     size_t off = 0;
     for (int i = 0; i < count; i++) {
-        int rc = fill_req(p, &p->req_h[i], ptms + off, n[i], m[i], ub[i], tags ? tags[i] : i);
+        int rc = fill_req(p, &p->req_h[i], ptms + off, n[i], m[i], ub[i], i);
         if (rc) return rc;
         off += (size_t)n[i] * m[i];
     }
@@ -377,7 +384,7 @@ extern "C" int pbb_pool_solve(pbb_pool *p, int count, const int32_t *ptms, const
     KArgs a = pool_kargs(p, budget);
     PoolArgs q = pool_pargs(p, false);
     if (!p->gexec || p->g_count != count || p->g_deadline != a.deadline || p->g_epi != p->epi ||
-        p->g_margin != p->margin) {
+        p->g_margin != p->margin || p->g_req != q.req_ticks) {
         if (p->gexec) {
             hipGraphExecDestroy(p->gexec);
             p->gexec = nullptr;
@@ -403,6 +410,7 @@ extern "C" int pbb_pool_solve(pbb_pool *p, int count, const int32_t *ptms, const
         HIPCHK(e5);
         p->g_count = count;
         p->g_deadline = a.deadline;
+        p->g_req = q.req_ticks;
         p->g_epi = p->epi;
         p->g_margin = p->margin;
     }
@@ -414,22 +422,43 @@ extern "C" int pbb_pool_solve(pbb_pool *p, int count, const int32_t *ptms, const
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, p->ev0, p->ev1));
     if (kernel_ms) *kernel_ms = ms;
+    p->last_count = count;
     int rc = check_run_error(p);
     if (rc) return rc;
     unsigned long long t0 = ULLONG_MAX;
     for (int i = 0; i < count; i++)
-        if (p->res_h[i].status != RES_PENDING) t0 = std::min(t0, p->res_h[i].t_admit);
+        if (p->res_h[i].tag == i) t0 = std::min(t0, p->res_h[i].t_admit);
     for (int i = 0; i < count; i++) {
-        if (p->res_h[i].tag != p->req_h[i].tag) {
+        const int tag = tags ? tags[i] : i;
+        if (p->res_h[i].tag != i) {
             // never retired (the launch ended first)
             PoolRes r{};
             r.status = RES_ABORTED;
             r.best = ub[i];
-            to_result(p, r, n[i], p->req_h[i].tag, t0, &out[i]);
+            to_result(p, r, n[i], tag, t0, &out[i]);
         } else {
-            to_result(p, p->res_h[i], n[i], p->req_h[i].tag, t0, &out[i]);
+            to_result(p, p->res_h[i], n[i], tag, t0, &out[i]);
         }
     }
+    return PBB_OK;
+}
+
+extern "C" int pbb_pool_batch_histograms(pbb_pool *p, int count, uint64_t *decomposed, uint64_t *refined)
+{
+    if (!p || count < 0 || (count > 0 && (!decomposed || !refined))) return fail(PBB_E_ARG, "bad arguments");
+    if (p->serving) return fail(PBB_E_STATE, "pool is serving");
+    if (count > p->last_count) return fail(PBB_E_ARG, "%d instances > the last batch's %d", count, p->last_count);
+    if (count == 0) return PBB_OK;
+    // a batch admits request i into slot i; each slot's control block keeps
+    // its histograms until the slot is admitted again
+    std::vector<Ctl> c(count);
+    HIPCHK(hipMemcpy2D(c.data(), sizeof(Ctl), p->arena + p->lay.hdr, p->lay.stride, sizeof(Ctl), count,
+                       hipMemcpyDeviceToHost));
+    for (int i = 0; i < count; i++)
+        for (int d = 0; d < 64; d++) {
+            decomposed[(size_t)i * 64 + d] = c[i].histo[d];
+            refined[(size_t)i * 64 + d] = c[i].jhisto[d];
+        }
     return PBB_OK;
 }
 

@@ -101,7 +101,7 @@ __device__ __forceinline__ InstRef slot_ref(const PoolArgs &p, int s, PoolSlot *
 // The whole workgroup builds slot s's tables from its request.  Machines
 // m..M-1 get zero times.  Returns false (uniformly) for a malformed request.
 template <int M>
-static __device__ bool pool_admit(const PoolArgs &p, int s, bool lb2)
+static __device__ __forceinline__ bool pool_admit(const PoolArgs &p, int s, bool lb2)
 {
     const int tid = threadIdx.x, nt = blockDim.x;
     char *b = slot_base(p, s);
@@ -217,7 +217,7 @@ static __device__ bool pool_admit(const PoolArgs &p, int s, bool lb2)
 // Wave 0 of a free workgroup: a new instance from the queue first, else the
 // running slot with the largest surplus of busy over idle explorers (below
 // the per-instance cap), else wait.  Returns the role; *slot receives the slot id.
-static __device__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_end, const int *host_flag, int *slot)
+static __device__ __forceinline__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_end, const int *host_flag, int *slot)
 {
     const int lane = lane_id();
     // A waiting group passes here every microsecond or so, and hundreds may
@@ -252,8 +252,10 @@ static __device__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_e
             const bool live = (o & POOL_RUNNING) && !(o & POOL_DONE);
             if (live) wc = ld_rlx(&slot_hdr(p, s)->c.work_count);
             const int mem = o & POOL_MEMBERS;
+            // a slot whose request ran out of time is draining: do not join it
+            const bool stopping = live && ld_rlx(&slot_hdr(p, s)->c.stop) != 0;
             // more busy than idle members: work is there to be split
-            const bool cand = live && mem + W <= p.epi && 2 * wc > mem;
+            const bool cand = live && !stopping && mem + W <= p.epi && 2 * wc > mem;
             running = running || ballot(live) != 0ull;
             const int sc = cand ? 2 * wc - mem : -1;
             const int mx = wave_max(sc);
@@ -278,8 +280,10 @@ static __device__ int pool_pick(const PoolArgs &p, int W, unsigned long long t_e
 }
 
 // ------------------------------------------------------------ retirement
-// the explorer whose release emptied the slot: record the optimum and its
-// schedule while every member is still inside, then mark the slot solved
+// the explorer whose release emptied the slot: record the incumbent and its
+// schedule while every member is still inside, then mark the slot done --
+// solved, or aborted when its stop word was raised (the request's time
+// budget ran out and its explorers gave their intervals up)
 template <int M, int REF>
 __device__ __forceinline__ void pool_finish(Explorer<M, REF> &X, const KArgs &a, PoolSlot *S)
 {
@@ -289,10 +293,11 @@ __device__ __forceinline__ void pool_finish(Explorer<M, REF> &X, const KArgs &a,
     const unsigned who = (unsigned)uni((int)(unsigned)(b & 0xffffffffull));
     const bool imp = who < (unsigned)a.E;
     if (imp && X.lane < X.n) S->perm[X.lane] = a.best_perm[(size_t)who * 64 + X.lane];
+    const bool stopped = uni(ld_rlx(&S->c.stop)) != 0;
     if (X.lane == 0) {
         S->best_cost = cost;
         if (imp) S->improved = 1;  // else the schedule given at admission (NEH), if any, stands
-        S->status = RES_SOLVED;
+        S->status = stopped ? RES_ABORTED : RES_SOLVED;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     if (X.lane == 0) {
@@ -302,7 +307,7 @@ __device__ __forceinline__ void pool_finish(Explorer<M, REF> &X, const KArgs &a,
 }
 
 // wave 0 of a leaving workgroup; the last member writes the result record
-static __device__ void pool_leave(const PoolArgs &p, int s, int W)
+static __device__ __forceinline__ void pool_leave(const PoolArgs &p, int s, int W)
 {
     const int lane = lane_id();
     PoolSlot *S = slot_hdr(p, s);
@@ -329,13 +334,15 @@ static __device__ void pool_leave(const PoolArgs &p, int s, int W)
 }
 
 // ----------------------------------------------------------------- kernel
-// Minimum resident waves per SIMD the compiler must allow (register budget).
+// Minimum resident waves per SIMD the compiler must allow (register
+// budget): 3 up to 10 machines (168 VGPRs: the 10-machine LB2 pool would
+// otherwise take 169-176 and drop to 2 waves per SIMD), free above.
 #ifndef PBB_POOL_MIN_WAVES
-#define PBB_POOL_MIN_WAVES 1
+#define PBB_POOL_MIN_WAVES(M) ((M) <= 10 ? 3 : 1)
 #endif
 
 template <int M, int REF>
-__global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolArgs p)
+__global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES(M)) void k_pool(KArgs a, PoolArgs p)
 {
     extern __shared__ __align__(16) char smem[];
     __shared__ int s_pick[2];
@@ -397,7 +404,17 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolA
             }
             const int ub = uni(ld_rlx(&S->ub));
             Explorer<M, REF> X(a, lds, e, slot_ref(p, slot, S, m, npairs, ub), p.run, n, I.mp);
+            // The request's own deadline (admission + its budget) and the
+            // launch's, whichever comes first, raise the slot's stop word: its
+            // explorers then give their intervals up as if exhausted, so the
+            // slot drains through the ordinary release path and every member
+            // leaves together once it is done (none leaves while a donor might
+            // still hand it work).
             X.t_end = t_end;
+            if (p.req_ticks) {
+                const unsigned long long dl = uni64(ld_rlx(&S->t_admit)) + p.req_ticks;
+                X.t_end = t_end ? (t_end < dl ? t_end : dl) : dl;
+            }
             X.local_best = uni((int)(unsigned)(ld_rlx(&S->best) >> 32));
             int st = ST_IDLE;
             if (role == ROLE_OWNER) {
@@ -464,7 +481,15 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES) void k_pool(KArgs a, PoolA
                     continue;
                 }
                 if (refresh) X.take_best(bw);
-                if ((++iter & 63u) == 0u && X.check_stop()) break;
+                if ((++iter & 63u) == 0u) {
+                    const int sc = X.check_stop_slot(&S->c.stop);
+                    if (sc == 1) break;  // the launch stops
+                    if (sc == 2) {       // this request stops: release the interval
+                        if (X.go_idle()) pool_finish(X, a, S);
+                        st = ST_IDLE;
+                        continue;
+                    }
+                }
                 if (poll) hungry = X.try_donate(rh, rt);
             }
             X.flush_counters();

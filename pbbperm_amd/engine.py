@@ -203,6 +203,9 @@ class SolveStats:
     depth_histo: List[int] = field(default_factory=list)
 
 
+DEFAULT_TIME_BUDGET_S = 3600.0
+
+
 class GpuBB:
     """The IVM branch-and-bound engine on one GPU (gpubb)."""
 
@@ -251,8 +254,9 @@ class GpuBB:
             raise ValueError("pos/end must hold nb*n factoradic digits")
         check(load().pbb_init_intervals(self._h, nb, _ip(ids), _ip(pos), _ip(end)))
 
-    # gpubb::next
-    def next(self, node_budget: int = 0, time_budget: float = 0.0) -> bool:
+    # gpubb::next.  time_budget 0 means none; the default bounds a call that
+    # nobody interrupts to an hour.
+    def next(self, node_budget: int = 0, time_budget: float = DEFAULT_TIME_BUDGET_S) -> bool:
         done = C.c_int(0)
         check(load().pbb_run(self._h, int(node_budget), float(time_budget), C.byref(done)))
         return bool(done.value)
@@ -309,7 +313,7 @@ class GpuBB:
         check(load().pbb_checkpoint_load(self._h, path.encode(), C.byref(c)))
         return {"explored": c.explored, "simple_bounds": c.simple_bounds, "best_cost": c.best_cost, "nb": c.nb}
 
-    def solve(self, node_budget: int = 0, time_budget: float = 0.0) -> bool:
+    def solve(self, node_budget: int = 0, time_budget: float = DEFAULT_TIME_BUDGET_S) -> bool:
         """init_full_interval + next until done (or budget). Returns True when finished."""
         self.init_full_interval()
         return self.next(node_budget, time_budget)

@@ -69,7 +69,7 @@ def default_params(**overrides) -> PoolParams:
     for k, v in overrides.items():
         if not hasattr(prm, k):
             raise KeyError(k)
-        setattr(prm, k, float(v) if k == "hbm_budget_gb" else int(v))
+        setattr(prm, k, float(v) if k in ("hbm_budget_gb", "request_budget_s") else int(v))
     return prm
 
 
@@ -121,6 +121,16 @@ class Pool:
                                        _ip(ub), None if tg is None else _ip(tg), float(time_budget), out,
                                        C.byref(kms)))
         return [_result(out[i], int(ns[i])) for i in range(len(ptms))], kms.value
+
+    def batch_histograms(self, count: int):
+        """Per-instance depth histograms of the last solve() batch:
+        (decomposed [count, 64], refined [count, 64]) as uint64 arrays."""
+        dec = np.zeros(max(1, count) * 64, dtype=np.uint64)
+        ref = np.zeros(max(1, count) * 64, dtype=np.uint64)
+        u64 = C.POINTER(C.c_uint64)
+        check(self._lib.pbb_pool_batch_histograms(self._h, int(count), dec.ctypes.data_as(u64),
+                                                  ref.ctypes.data_as(u64)))
+        return dec[:count * 64].reshape(count, 64), ref[:count * 64].reshape(count, 64)
 
     # ------------------------------------------------------------ serving
     def serve_start(self, time_budget: float):
@@ -203,8 +213,9 @@ def replay_routed(pools: Sequence["Pool"], ptms: Sequence[np.ndarray], ubs: Sequ
 
 
 def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Sequence[float],
-                 config: Optional[Config] = None, time_budget: float = 0.0):
-    """The same stream on the one-instance engine, one launch per instance, FCFS."""
+                 config: Optional[Config] = None, time_budget: float = 0.0, request_budget: float = 0.0):
+    """The same stream on the one-instance engine, one launch per instance, FCFS
+    (request_budget: per-instance time cap, as the pool's request_budget_s)."""
     cfg = config if config is not None else default_config()
     flat, ns, ms = _pack(ptms)
     ub = _lib.sized(ubs, np.int32, len(ptms), "ubs")
@@ -213,7 +224,8 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
     lat = np.zeros(max(1, len(ptms)), dtype=np.float64)
     wall = C.c_double()
     check(load().pbb_plain_replay(C.byref(cfg), len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms),
-                                  _ip(ub), _dp(arr), float(time_budget), out, _dp(lat), C.byref(wall)))
+                                  _ip(ub), _dp(arr), float(time_budget), float(request_budget), out, _dp(lat),
+                                  C.byref(wall)))
     return [_result(out[i], int(ns[i])) for i in range(len(ptms))], lat[:len(ptms)], wall.value
 
 
@@ -324,22 +336,10 @@ def batch_stream(count: int = 512, seed0: int = 100) -> List[np.ndarray]:
     return instance_stream(range(seed0, seed0 + count), 10, 22, (5, 10))
 
 
-SERVE_NMAX_M20 = 16
-
-
 def serving_stream(count: int, seed0: int = 10000) -> List[np.ndarray]:
     """The serving workload: request i has seed seed0+i, n ~ U[12,30] and
-    m ~ U{5,10,20} from its shape stream.  Requests with 20 machines are
-    folded to n in [12,16] (n = 12 + (n - 12) mod 5): 20-machine instances of
-    this generator with n >= 17 need more than 3M nodes with LB1 + LB2
-    (35 of 39 sampled on the CPU oracle), far beyond a request's latency."""
-    out = []
-    for s in range(seed0, seed0 + count):
-        n, m = synth_shape(s, 12, 30, (5, 10, 20))
-        if m == 20:
-            n = 12 + (n - 12) % (SERVE_NMAX_M20 - 11)
-        out.append(synth(s, n, m))
-    return out
+    m ~ U{5,10,20} from its shape stream, times from its PTM stream."""
+    return instance_stream(range(seed0, seed0 + count), 12, 30, (5, 10, 20))
 
 
 def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
@@ -366,20 +366,22 @@ WORKLOADS = {
 }
 
 # The serving stream split over two pools resident together (replay_routed):
-# 5- and 10-machine requests on a 10-machine pool, 20-machine requests
-# (n <= 16) on a 20-machine pool.  Group counts from a sweep on the
-# 600-request stream at 30 requests/s on one MI355X (p50 / mean latency,
-# ms): 576+128 groups 13.5 / 52.2, 512+192 11.2 / 49.5, 384+256 21.4 / 157;
-# one 20-machine pool 26.3-26.5 / 59.6-73.5.
+# 5- and 10-machine requests on a 10-machine pool, 20-machine requests on a
+# 20-machine pool, both for n <= 30.  The 20-machine pool starts first (its
+# waves need the most registers) and keeps 192 groups; the 10-machine pool
+# takes 512 groups beside it (a split first measured on a stream whose
+# 20-machine requests had n <= 16: 576+128 groups gave p50 / mean latency
+# 13.5 / 52.2 ms, 512+192 11.2 / 49.5, 384+256 21.4 / 157).
 SERVING_ROUTED = (dict(machines=10, max_jobs=30, groups=512),
-                  dict(machines=20, max_jobs=SERVE_NMAX_M20, groups=192))
+                  dict(machines=20, max_jobs=30, groups=192))
 
 
 def serving_routed_params(base: PoolParams) -> List[PoolParams]:
     """PoolParams of the routed serving pools, with base's tuned waves per
     group, explorers per instance and steal margin."""
     keep = dict(slots=base.slots, waves_per_group=base.waves_per_group,
-                explorers_per_instance=base.explorers_per_instance, steal_margin=base.steal_margin)
+                explorers_per_instance=base.explorers_per_instance, steal_margin=base.steal_margin,
+                request_budget_s=base.request_budget_s)
     return [default_params(**keep, **shape) for shape in SERVING_ROUTED]
 
 

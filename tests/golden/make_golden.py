@@ -11,6 +11,8 @@ tree exists.  The fixtures are plain data (inputs and expected outputs):
   solve_cases.json            optimum / decomposed / leaves / LB1 bounds /
                               refinement bounds of single-thread solves
   taillard_ptm.json           sha256 of the ta001..ta120 processing times
+  findall_cases.json          single-thread solves with findAll (prune on
+                              >, the bound frozen at the optimum)
   neh_cases.json              the reference fastNEH (initialSort + runNEH
                               from the identity order, its uninitialised
                               tail cell zeroed by the harness) on the 20x5
@@ -20,7 +22,7 @@ tree exists.  The fixtures are plain data (inputs and expected outputs):
                               to tie: equal job totals and equal insertion
                               makespans (explicit times)
 
-Usage: python tests/golden/make_golden.py [bounds|solve|taillard|neh ...]
+Usage: python tests/golden/make_golden.py [bounds|solve|taillard|neh|findall ...]
 """
 from __future__ import annotations
 
@@ -116,12 +118,39 @@ def neh_fixture():
     print("neh cases", len(rows), flush=True)
 
 
+# findAll (multicore/src/ivm_bound.cpp:678-682): pruning with > instead of
+# >=, with the bound frozen at the optimum, so every optimal schedule's branch
+# is explored.  (seed, n, m, bound_mode, branching, refine, johnson_pairs)
+FINDALL_CASES = [(9100 + s, 7 + s % 4, 3 + s % 3, bm, br, rf, jp)
+                 for s in range(8)
+                 for (bm, br, rf, jp) in [(0, 1, 0, 0), (0, 2, 0, 0), (0, 3, 0, 0), (0, -1, 0, 0),
+                                          (2, 1, 0, 0), (2, 3, 1, 0), (1, 1, 0, 0), (1, 3, 1, 2)]]
+
+
+def findall_fixture():
+    rows = []
+    for (seed, n, m, bm, br, rf, jp) in FINDALL_CASES:
+        ptm = O.synth_ptm(seed, n, m)
+        neh, _ = O.Instance(ptm).neh()
+        opt = O.ref_solve(ptm, neh, 0, 1).best
+        r = O.ref_solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp, find_all=True)
+        assert r.finished
+        rows.append(dict(seed=seed, n=n, m=m, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp,
+                         ub=opt, best=r.best, decomposed=r.decomposed, leaves=r.leaves,
+                         simple_bounds=r.simple_bounds, johnson_bounds=r.johnson_bounds))
+    with open(os.path.join(HERE, "findall_cases.json"), "w") as f:
+        json.dump(rows, f, indent=0)
+    print("findall cases", len(rows), flush=True)
+
+
 def main():
     if not O.have_ref():
         O.build(ref=True)
-    which = set(sys.argv[1:]) or {"bounds", "solve", "taillard", "neh"}
+    which = set(sys.argv[1:]) or {"bounds", "solve", "taillard", "neh", "findall"}
     if "neh" in which:
         neh_fixture()
+    if "findall" in which:
+        findall_fixture()
     if not which & {"bounds", "solve", "taillard"}:
         return
     for seed, n, m, count in BOUND_SHAPES:

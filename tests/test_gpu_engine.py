@@ -67,6 +67,26 @@ def test_small_instances_all_modes_frozen_ub(oracle_mod, seed):
         assert _same_counts(st, ref), (st, ref)
 
 
+def test_find_all_frozen_ub_matches_reference():
+    """findAll (prune on >, multicore/src/ivm_bound.cpp:678-682) with the
+    bound frozen at the optimum, every bounding/branching mode of the
+    reference fixture: counts equal the reference's single-thread run, for a
+    single explorer and for many."""
+    import json
+    import os
+    import pbbperm_amd as P
+    with open(os.path.join(os.path.dirname(__file__), "golden", "findall_cases.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        ptm = P.synth(c["seed"], c["n"], c["m"])
+        want = (c["best"], c["decomposed"], c["leaves"], c["simple_bounds"], c["johnson_bounds"])
+        for nb in (1, 4096):
+            best, _, st = _gpu(P, ptm, c["ub"], c["bound_mode"], c["branching"], c["refine"], c["johnson_pairs"],
+                               nb=nb, find_all=1)
+            got = (best, st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds)
+            assert got == want, (c, nb, got)
+
+
 @pytest.mark.parametrize("seed,n,m", [(42, 16, 8), (3, 16, 6)])
 def test_larger_trees_frozen_ub(oracle_mod, seed, n, m):
     """Trees of 10^4-10^6 nodes: work sharing is exercised for real."""
@@ -351,8 +371,16 @@ def test_checkpoint_save_load_resume(oracle_mod, tmp_path):
         info = eng2.load_checkpoint(path)
         assert info["explored"] == first.decomposed and info["best_cost"] == opt
         assert info["simple_bounds"] == first.simple_bounds
+        # the file's counters carry on into the engine (works::init_infile)
+        assert eng2.stats().decomposed == first.decomposed
+        assert eng2.stats().simple_bounds == first.simple_bounds
         assert eng2.next(time_budget=60.0)
-        assert info["explored"] + eng2.stats().decomposed == expect
+        assert eng2.stats().decomposed == expect
+        # a second save holds the whole run's counts, not the last segment's
+        eng2.save_checkpoint(path + ".2")
+        with open(path + ".2") as f:
+            hdr2, _, _, _ = P.checkpoint_parse(f.read(), n)
+        assert hdr2["explored"] == expect and hdr2["nb"] == 0
         assert eng2.best()[0] == opt
         eng2.close()
 

@@ -384,6 +384,92 @@ def test_device_neh_matches_host_neh():
         assert P.makespan(p, o) == hc
 
 
+def test_pool_find_all_matches_reference():
+    """findAll in the pool (all instances of the fixture in one launch, each
+    with its own mode's pool): counts equal the reference's."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    with open(os.path.join(HERE, "golden", "findall_cases.json")) as f:
+        cases = json.load(f)
+    modes = sorted({(c["bound_mode"], c["branching"], c["refine"], c["johnson_pairs"]) for c in cases})
+    for mode in modes:
+        sel = [c for c in cases if (c["bound_mode"], c["branching"], c["refine"], c["johnson_pairs"]) == mode]
+        cfg = _cfg(P, *mode)
+        cfg.find_all = 1
+        ptms = [P.synth(c["seed"], c["n"], c["m"]) for c in sel]
+        with PP.Pool(cfg, max_jobs=12, machines=5, slots=len(sel), waves_per_group=2,
+                     explorers_per_instance=32) as pool:
+            res, _ = pool.solve(ptms, [c["ub"] for c in sel], time_budget=BUDGET)
+        for c, r in zip(sel, res):
+            assert r.status == "solved" and r.best == c["best"]
+            assert _counts(r) == [c["decomposed"], c["leaves"], c["simple_bounds"], c["johnson_bounds"]], (c, r)
+
+
+def test_pool_tags_are_returned_and_never_decide_retirement(oracle_mod):
+    """Caller tags come back as given, -1 included; whether an instance
+    retired is decided by the request's index, so with a deadline that ends
+    the launch first the unfinished instances read ABORTED whatever their
+    tag (a tag of -1 once equalled the reset value of an unwritten result)."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    O = oracle_mod
+    ptms = _small_batch(O, 6, 7800)
+    opts = [_opt(O, p) for p in ptms]
+    tags = [-1, 5, -1, 7, 123456, -1]
+    with PP.Pool(_cfg(P, 0, 1, 0, 0), max_jobs=12, machines=5, slots=8, waves_per_group=2,
+                 explorers_per_instance=32) as pool:
+        res, _ = pool.solve(ptms, opts, tags=tags, time_budget=BUDGET)
+    assert [r.tag for r in res] == tags
+    assert all(r.status == "solved" and r.best == u for r, u in zip(res, opts))
+    # a 30x20 instance from infinity cannot finish in 0.2 s: it must read aborted
+    big = [P.synth(31, 30, 20)] + ptms[:2]
+    with PP.Pool(_cfg(P, 0, 1, 0, 0), max_jobs=30, machines=20, slots=4, waves_per_group=2,
+                 explorers_per_instance=64) as pool:
+        res, _ = pool.solve(big, [2**31 - 1, opts[0], opts[1]], tags=[-1, -1, -1], time_budget=0.2)
+    assert res[0].status == "aborted" and res[0].tag == -1
+    assert res[1].status == "solved" and res[2].status == "solved"
+
+
+def test_request_budget_retires_long_requests_and_keeps_the_rest_exact(oracle_mod):
+    """A per-request time budget: requests whose trees cannot finish within
+    it (30x20 from an infinite incumbent) retire ABORTED shortly after it,
+    with a schedule of the reported cost when one was found, while the small
+    requests around them keep exact optima and frozen-bound counts; the slot
+    of an aborted request serves later requests.  Serving and batch mode."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    O = oracle_mod
+    small = _small_batch(O, 24, 7900)
+    opts = [_opt(O, p) for p in small]
+    want = [O.solve(p, u, bound_mode=0, branching=1) for p, u in zip(small, opts)]
+    big = [P.synth(41 + i, 30, 20) for i in range(3)]
+    ptms, ubs, kinds = [], [], []
+    for i, (p, u) in enumerate(zip(small, opts)):
+        ptms.append(p); ubs.append(u); kinds.append(i)
+        if i % 8 == 3:
+            ptms.append(big[i // 8]); ubs.append(2**31 - 1); kinds.append(-1)
+    budget = 0.25
+    with PP.Pool(_cfg(P, 0, 1, 0, 0), max_jobs=30, machines=20, slots=8, waves_per_group=2,
+                 explorers_per_instance=64, request_budget_s=budget) as pool:
+        arrivals = np.linspace(0.0, 0.5, len(ptms))
+        res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=BUDGET)
+        for r, l, k, p in zip(res, lat, kinds, ptms):
+            if k < 0:
+                assert r.status == "aborted", r
+                assert budget * 1e3 <= l < budget * 1e3 + 2000, l
+                _check_schedule(P, p, r)
+            else:
+                w = want[k]
+                assert r.status == "solved" and r.best == opts[k]
+                assert _counts(r) == [w.decomposed, w.leaves, w.simple_bounds, w.johnson_bounds]
+        res, _ = pool.solve(ptms[:5], ubs[:5], time_budget=BUDGET)
+        assert [r.status for r in res] == ["solved"] * 4 + ["aborted"]
+        assert [r.best for r in res[:4]] == opts[:4]
+    plain, plat, _ = PP.plain_replay(ptms[:5], ubs[:5], np.zeros(5), config=_cfg(P, 0, 1, 0, 0),
+                                     time_budget=BUDGET, request_budget=budget)
+    assert [r.status for r in plain] == ["solved", "solved", "solved", "aborted", "solved"]
+
+
 def test_device_neh_matches_reference_fixture(oracle_mod):
     """pbb_neh_device gives the reference fastNEH's order and makespan on the
     committed fixture (20x5, the 512 batched instances, the serving stream,
@@ -414,4 +500,11 @@ def test_pool_computes_neh_at_admission(oracle_mod):
     for p, r, u in zip(ptms, res, opts):
         assert r.status == "solved" and r.best == u
         assert r.improved
+        _check_schedule(P, p, r)
+    # the plain path takes the same stream: a UB_NEH request gets the host
+    # NEH's incumbent (never a cost of -1 that would prune every child)
+    plain, _, _ = PP.plain_replay(ptms[:8], [PP.UB_NEH] * 8, np.zeros(8), config=_cfg(P, 2, 3, 1, 0),
+                                  time_budget=BUDGET)
+    for p, r, u in zip(ptms[:8], plain, opts[:8]):
+        assert r.status == "solved" and r.best == u and r.improved
         _check_schedule(P, p, r)

@@ -199,20 +199,28 @@ def test_routed_replay_rejects_bad_pool_lists():
 
 
 def test_serving_routed_params_keep_the_tuned_fields():
-    """The routed serving pools: a 10-machine pool for n <= 30 and a
-    20-machine pool for the folded 20-machine requests, both with the base
-    parameters' tuned waves per group, explorers per instance and margin."""
+    """The routed serving pools: a 10-machine and a 20-machine pool, both for
+    n <= 30, with the base parameters' tuned waves per group, explorers per
+    instance, margin and request budget."""
     import pbbperm_amd.pool as PP
-    base = PP.workload_params("serving", waves_per_group=2, explorers_per_instance=512, steal_margin=4)
+    base = PP.workload_params("serving", waves_per_group=2, explorers_per_instance=512, steal_margin=4,
+                              request_budget_s=7.5)
     p10, p20 = PP.serving_routed_params(base)
-    assert (p10.machines, p10.max_jobs, p20.machines, p20.max_jobs) == (10, 30, 20, PP.SERVE_NMAX_M20)
+    assert (p10.machines, p10.max_jobs, p20.machines, p20.max_jobs) == (10, 30, 20, 30)
+    assert p10.request_budget_s == p20.request_budget_s == 7.5
     assert p10.groups > 0 and p20.groups > 0
     for q in (p10, p20):
         assert (q.waves_per_group, q.explorers_per_instance, q.steal_margin, q.slots) == (2, 512, 4, base.slots)
-    # every request of the serving stream fits one of the two pools
-    for ptm in PP.serving_stream(200, 10000):
+    # every request of the serving stream fits one of the two pools, and the
+    # stream is not folded: n spans 12..30 at every machine count
+    seen = {}
+    for ptm in PP.serving_stream(600, 10000):
         m, n = ptm.shape
         assert any(m <= q.machines and n <= q.max_jobs for q in (p10, p20)), (n, m)
+        seen.setdefault(m, set()).add(n)
+    assert sorted(seen) == [5, 10, 20]
+    for m, ns in seen.items():
+        assert min(ns) == 12 and max(ns) == 30, (m, sorted(ns))
 
 
 def test_engine_fails_loudly_without_device():
@@ -222,3 +230,41 @@ def test_engine_fails_loudly_without_device():
         pytest.skip("a GPU is present")
     with pytest.raises(P.PbbError):
         P.GpuBB(P.synth(0, 10, 3))
+
+
+def test_option_checks_before_any_device_work(tmp_path):
+    """Option values the engine cannot honour are refused with PBB_E_ARG by
+    the ini parser and by pbb_create before it touches a device: a negative
+    incumbent (PBB_UB_NEH belongs to the pool entry points), an incumbent
+    order that is not a permutation, a negative steal margin, and
+    earlyStopJohnson with boundingMode 1 and the Johnson refinement (its
+    early values would choose the branching direction,
+    bounds/src/bound_fsp_strong.cpp:150-154, multicore/src/ivm_bound.cpp:668-676).
+    The same option with boundingMode 2 only changes when an already-pruned
+    child's bound stops, so it is accepted."""
+    import pbbperm_amd as P
+    from pbbperm_amd import _lib
+    lib = P.load()
+    ptm = np.ascontiguousarray(P.synth(1, 8, 3), dtype=np.int32)
+    h = C.c_void_p()
+
+    def create(cfg, ub, perm=None):
+        p = None if perm is None else np.ascontiguousarray(perm, dtype=np.intc)
+        return lib.pbb_create(ptm.ctypes.data_as(_lib.I32P), 8, 3, C.byref(cfg), ub,
+                              None if p is None else p.ctypes.data_as(_lib.IP), C.byref(h))
+
+    ok = P.default_config(bound_mode=0, early_stop_johnson=0)
+    assert create(ok, -1) == -1 and "negative" in lib.pbb_last_error().decode()
+    assert create(ok, 500, [0, 1, 2, 3, 4, 5, 6, 6]) == -1
+    assert create(P.default_config(steal_margin=-1), 500) == -1
+    bad = P.default_config(bound_mode=1, refine=1, early_stop_johnson=1)
+    assert create(bad, 500) == -1 and "earlyStopJohnson" in lib.pbb_last_error().decode()
+    ini = tmp_path / "gpuconfig.ini"
+    ini.write_text("[bb]\nsortedDFS = 0\nboundingMode = 1\nrefinementBound = 1\nearlyStopJohnson = true\n")
+    with pytest.raises(P.PbbError):
+        P.load_config(str(ini))
+    for text in ("boundingMode = 2\nrefinementBound = 1\nearlyStopJohnson = true\n",
+                 "boundingMode = 1\nrefinementBound = 0\nearlyStopJohnson = true\n",
+                 "boundingMode = 1\nrefinementBound = 1\nearlyStopJohnson = false\n"):
+        ini.write_text("[bb]\nsortedDFS = 0\n" + text)
+        P.load_config(str(ini))

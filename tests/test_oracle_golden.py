@@ -139,3 +139,23 @@ def test_host_neh_matches_reference(oracle_mod):
     for c, ptm in _neh_cases(oracle_mod):
         cost, perm = P.neh(ptm)
         assert (cost, list(perm)) == (c["cost"], c["perm"]), (c["kind"], c["seed"])
+
+
+def test_oracle_findall_solves_match_reference(oracle_mod):
+    """findAll (prune on >, multicore/src/ivm_bound.cpp:678-682) with the
+    bound frozen at the optimum: the oracle's counts equal the reference's,
+    and the fixture tells the two pruning rules apart (>= gives fewer nodes
+    on most cases)."""
+    with open(os.path.join(GOLDEN, "findall_cases.json")) as f:
+        cases = json.load(f)
+    differ = 0
+    for c in cases:
+        ptm = oracle_mod.synth_ptm(c["seed"], c["n"], c["m"])
+        kw = dict(bound_mode=c["bound_mode"], branching=c["branching"], refine=c["refine"],
+                  johnson_pairs=c["johnson_pairs"])
+        r = oracle_mod.solve(ptm, c["ub"], find_all=True, **kw)
+        got = (r.best, r.decomposed, r.leaves, r.simple_bounds, r.johnson_bounds)
+        want = (c["best"], c["decomposed"], c["leaves"], c["simple_bounds"], c["johnson_bounds"])
+        assert got == want, (c, got)
+        differ += oracle_mod.solve(ptm, c["ub"], **kw).decomposed != c["decomposed"]
+    assert differ > len(cases) // 2
