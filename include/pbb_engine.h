@@ -314,6 +314,23 @@ int pbb_pool_batch_histograms(pbb_pool *p, int count, uint64_t *decomposed, uint
  * perms[64 i .. 64 i + n[i] - 1]; costs: count makespans. */
 int pbb_neh_device(int count, const int32_t *ptms, const int *n, const int *m, int *perms, int *costs);
 
+/* pbb_neh_device on buffers already in device memory, allocating nothing.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  d_ptms,
+ * d_n, d_m, d_perms (count x 64 ints) and d_costs are device pointers laid
+ * out as in pbb_neh_device; max_m (2..20) bounds every m[i] and picks the
+ * kernel.  Asynchronous on `stream` (a hipStream_t, NULL = default stream).
+ * The instances' offsets go to a fixed device array of PBB_NEH_NOALLOC_MAX
+ * entries (256 KB per device, held until the process ends) shared by every
+ * call on the device: count must not exceed PBB_NEH_NOALLOC_MAX
+ * (PBB_E_CAPACITY), and calls that overlap on one device race.  An instance
+ * with n outside [3, 63] or m outside [2, max_m] gets cost -1 and order -1;
+ * its n*m times still count in the offsets of the later instances when
+ * 1 <= n <= 63 and 1 <= m <= 20 (other sizes count as 0). */
+#define PBB_NEH_NOALLOC_MAX 32768
+int pbb_neh_device_noalloc(int count, const int32_t *d_ptms, const int *d_n, const int *d_m, int max_m,
+                           int *d_perms, int *d_costs, void *stream);
+
 /* Batch: count <= slots instances, instance i's m[i] x n[i] machine-major
  * times at ptms[off_i], off_i = sum of n[k]*m[k] for k < i.  Upload, reset,
  * the resident launch and the result download are one graph launch.

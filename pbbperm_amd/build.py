@@ -47,6 +47,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     for src in ("pbb_dispatch.cpp", "pbb_host.cpp", "pbb_pool.cpp", "pbb_checkpoint.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         units.append([HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
+    # compiled once: it defines the one device array of the allocation-free entry point
+    units.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, "pbb_noalloc.hip"),
+                  "-o", os.path.join(BUILD, "pbb_noalloc.o")])
     with cf.ThreadPoolExecutor(jobs) as ex:
         errs = [e for e in ex.map(_run, units) if e]
     if errs:

@@ -529,7 +529,10 @@ __global__ void k_pool_init(PoolArgs p, int count)
 }
 
 // NEH for many instances, one wavefront each (times [m][n] at ptms + off[i],
-// padded to M machines with zero times, which changes no partial makespan)
+// padded to M machines with zero times, which changes no partial makespan).
+// A shape outside 3 <= n <= 63, 2 <= m <= M gets cost -1 and order -1 (the
+// host checks shapes before the allocating entry point launches this; the
+// allocation-free one leaves the check to the device).
 template <int M>
 __global__ __launch_bounds__(64) void k_neh(const int *ptms, const long long *off, const int *ns, const int *ms,
                                             int count, int nmax, int *perms, int *costs)
@@ -538,6 +541,11 @@ __global__ __launch_bounds__(64) void k_neh(const int *ptms, const long long *of
     const int s = blockIdx.x;
     if (s >= count) return;
     const int n = ns[s], m = ms[s], lane = lane_id();
+    if (n < 3 || n > PBB_POOL_ROW - 1 || m < 2 || m > M) {
+        perms[(size_t)s * 64 + lane] = -1;
+        if (lane == 0) costs[s] = -1;
+        return;
+    }
     const int ps = lds_pm_stride(nmax);
     int *pm = (int *)smem;
     int *seq = pm + M * ps;
