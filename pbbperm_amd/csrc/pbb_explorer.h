@@ -5,6 +5,8 @@
 // function is force-inlined into the kernels that use it.
 #pragma once
 
+#include <utility>
+
 #include "pbb_device.h"
 #include "pbb_kernels.h"
 
@@ -381,8 +383,9 @@ struct Explorer {
     // the same form as the front one run over 0..M-1, so both halves execute
     // one loop:  t1 = max(x_q, t0); lb = max(lb, t1 + y_q); t0 = t1 + p_q,
     // with (x_q, y_q) = (front_q, remain_q + back_q) in the front half and
-    // (back_{M-1-q}, front_{M-1-q} + remain_{M-1-q}) in the back half, read
-    // from the per-wave operand block as LDS broadcasts.
+    // (back_{M-1-q}, front_{M-1-q} + remain_{M-1-q}) in the back half.
+    // The per-wave operand block U holds the front half's pairs in machine
+    // order, then the back half's.
     __device__ __forceinline__ void build_operands() const
     {
         if (lane < M) {
@@ -393,11 +396,52 @@ struct Explorer {
         __builtin_amdgcn_wave_barrier();
     }
 
-    // one fused pass over children base..base+31; returns this lane's bound
-    // (chain receives t0 after every step when non-null: the child's own
-    // front chain in machine order (front half) or back chain in reversed
-    // machine order (back half))
-    __device__ __forceinline__ int lb1_pass(int base, int k, int *chain) const
+    // lane Q of every 16-lane row, to all lanes of the row (DPP row_newbcast,
+    // folded into the instruction that uses it)
+    template <int Q>
+    __device__ __forceinline__ static int row_bcast_lane(int v)
+    {
+        return __builtin_amdgcn_update_dpp(0, v, 0x150 + (Q & 15), 0xF, 0xF, false);
+    }
+
+    // machine step Q of the fused pass; the step's operand pair is lane Q mod
+    // 16 of the row's copy of U (A for Q < 16, A2 above)
+    template <int Q>
+    __device__ __forceinline__ static void lb1_step(const int2 &A, const int2 &A2, const int (&pj)[M], int &t0,
+                                                    int &lb)
+    {
+        int x, y;
+        if constexpr (Q < 16) {
+            x = row_bcast_lane<Q>(A.x);
+            y = row_bcast_lane<Q>(A.y);
+        } else {
+            x = row_bcast_lane<Q - 16>(A2.x);
+            y = row_bcast_lane<Q - 16>(A2.y);
+        }
+        if constexpr (Q == 0) {
+            t0 = x + pj[0];
+            lb = x + y;
+        } else {
+            const int t1 = max(x, t0);
+            lb = max(lb, t1 + y);
+            t0 = t1 + pj[Q];
+        }
+    }
+
+    template <int... Q>
+    __device__ __forceinline__ static int lb1_steps(const int2 &A, const int2 &A2, const int (&pj)[M],
+                                                    std::integer_sequence<int, Q...>)
+    {
+        int t0 = 0, lb = 0;
+        (lb1_step<Q>(A, A2, pj, t0, lb), ...);
+        return lb;
+    }
+
+    // one fused pass over children base..base+31; returns this lane's bound.
+    // Each 16-lane row reads its half's operand pairs once (two vector LDS
+    // reads, no bank conflicts) and every machine step takes its pair from
+    // lane q of the row by DPP, instead of M broadcast reads of U.
+    __device__ __forceinline__ int lb1_pass(int base, int k) const
     {
         const int half = lane >> 5;
         const int c = base + (lane & 31);
@@ -405,35 +449,27 @@ struct Explorer {
         const int jb = c < k ? job : 0;
         // machine-major rows: front lanes walk machines 0..M-1, back lanes M-1..0
         const int *pcol = (half ? lds.pmR : lds.pm) + jb;
+        const int ql = lane & 15;
         const int2 *u = lds.U + half * M;
+        const int2 A = u[min(ql, M - 1)];
+        int2 A2 = make_int2(0, 0);
+        if constexpr (M > 16) A2 = u[16 + min(ql, M - 17)];
         int pj[M];
 #pragma unroll
         for (int q = 0; q < M; q++) pj[q] = pcol[q * PM_STRIDE];
-        int2 o = u[0];
-        int t0 = o.x + pj[0];
-        int lb = o.x + o.y;
-        if (chain) chain[0] = t0;
-#pragma unroll
-        for (int q = 1; q < M; q++) {
-            o = u[q];
-            int t1 = max(o.x, t0);
-            lb = max(lb, t1 + o.y);
-            t0 = t1 + pj[q];
-            if (chain) chain[q] = t0;
-        }
-        return lb;
+        return lb1_steps(A, A2, pj, std::make_integer_sequence<int, M>{});
     }
 
     // row layout (lane = child): cb (front) and ce (back) for k <= 64 children
-    __device__ __forceinline__ void lb1_children(int k, int &cb, int &ce, int *chain0) const
+    __device__ __forceinline__ void lb1_children(int k, int &cb, int &ce) const
     {
         build_operands();
-        int lb0 = lb1_pass(0, k, chain0);
+        int lb0 = lb1_pass(0, k);
         if (k <= 32) {
             cb = lb0;
             ce = bcast_high_half(lb0);
         } else {
-            int lb1 = lb1_pass(32, k, nullptr);
+            int lb1 = lb1_pass(32, k);
             int sh0 = bcast_high_half(lb0), sh1 = bcast_low_half(lb1);
             cb = lane < 32 ? lb0 : sh1;
             ce = lane < 32 ? sh0 : lb1;
@@ -486,7 +522,7 @@ struct Explorer {
         bool refined = false;
         if constexpr (REF == 0) {
             int cb, ce;
-            lb1_children(k, cb, ce, nullptr);
+            lb1_children(k, cb, ce);
             if (!both) ce = 0;  // costsEnd[WEAK] stays zero (boundChildren front only)
             sb += (unsigned)(both ? 2 * k : k);
             dir = choose(valid, cb, ce);
@@ -494,7 +530,7 @@ struct Explorer {
         } else {
             const int mode = root ? 0 : a.bound_mode;
             int cb, ce;
-            lb1_children(k, cb, ce, nullptr);
+            lb1_children(k, cb, ce);
             if (mode == 1) {
                 int csF = 0, csB = 0;
                 refine_both(valid, job, csF, csB);

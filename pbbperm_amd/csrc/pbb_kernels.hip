@@ -227,7 +227,7 @@ __global__ __launch_bounds__(256) void k_bounds(KArgs a, int which, int count, c
         X.rowv = job;
         X.L = n - k;
         int cb, ce;
-        X.lb1_children(k, cb, ce, nullptr);
+        X.lb1_children(k, cb, ce);
         int *inv = lds.scr;
         if (lane < n) inv[pv] = lane;
         __builtin_amdgcn_wave_barrier();

@@ -467,7 +467,8 @@ def test_request_budget_retires_long_requests_and_keeps_the_rest_exact(oracle_mo
         assert [r.best for r in res[:4]] == opts[:4]
     plain, plat, _ = PP.plain_replay(ptms[:5], ubs[:5], np.zeros(5), config=_cfg(P, 0, 1, 0, 0),
                                      time_budget=BUDGET, request_budget=budget)
-    assert [r.status for r in plain] == ["solved", "solved", "solved", "aborted", "solved"]
+    assert [r.status for r in plain] == ["solved"] * 4 + ["aborted"]
+    assert [r.best for r in plain[:4]] == opts[:4]
 
 
 def test_device_neh_matches_reference_fixture(oracle_mod):
