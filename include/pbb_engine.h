@@ -199,6 +199,49 @@ int pbb_get_deep_subproblems(pbb_engine *e, int *perms, int max, int *count);
  * running kernel within a few thousand nodes. */
 int pbb_interrupt(pbb_engine *e);
 
+/* ------------------------------------------------ host co-worker
+ * IVM explorers on host threads (the reference's multicore explorers,
+ * multicore/src/matrix_controller.cpp and bbthread.cpp, as a worker of
+ * distributed/src/worker_mc.cpp:26-111) on the same work units as the
+ * engine: pbb_cpu_init_intervals / pbb_cpu_get_intervals take and return
+ * the factoradic intervals that pbb_get_intervals / pbb_init_intervals
+ * exchange, so idle host cores explore intervals moved off the GPU.  Every
+ * node is processed with the engine's semantics (bounds, branching, pruning,
+ * counters): with a frozen incumbent an interval's node counts are the same
+ * on either side.  Threads share work among themselves as bbthread::shareWork
+ * does. */
+typedef struct pbb_cpu pbb_cpu;
+
+/* threads = 0: one per hardware thread.  ub / ub_perm as pbb_create. */
+int pbb_cpu_create(const int32_t *ptm, int n, int m, const pbb_config *cfg, int threads, int ub,
+                   const int *ub_perm, pbb_cpu **out);
+int pbb_cpu_destroy(pbb_cpu *c);
+int pbb_cpu_set_root(pbb_cpu *c, const int *root_perm);
+/* matrix_controller::initFromFac (multicore/src/matrix_controller.cpp:112-140):
+ * interval k to thread ids[k] < threads, unfolded (bounded, uncounted). */
+int pbb_cpu_init_intervals(pbb_cpu *c, int nb, const int *ids, const int *pos, const int *end);
+int pbb_cpu_init_full(pbb_cpu *c);
+/* explore until every thread is dry (*all_done = 1), node_budget decomposed
+ * nodes, time_budget_s seconds (0 = none) or pbb_cpu_interrupt */
+int pbb_cpu_run(pbb_cpu *c, uint64_t node_budget, double time_budget_s, int *all_done);
+/* matrix_controller::getIntervals (:142-170); counters as pbb_get_intervals */
+int pbb_cpu_get_intervals(pbb_cpu *c, int *pos, int *end, int *ids, int *nb, int max, uint64_t *decomposed,
+                          uint64_t *leaves);
+int pbb_cpu_get_incumbent(pbb_cpu *c, int *cost, int *perm);
+int pbb_cpu_set_incumbent(pbb_cpu *c, int cost, const int *perm);
+int pbb_cpu_get_stats(pbb_cpu *c, pbb_stats *st);
+int pbb_cpu_interrupt(pbb_cpu *c);
+
+/* The engine and the host co-worker on one search (no counterpart in the
+ * reference, which runs them as separate workers of its master): periods of
+ * period_s seconds run both at once; whenever the host threads are dry, a
+ * quarter of the engine's intervals (at most one per thread) moves to them;
+ * incumbents are exchanged after every period.  Ends when both are done or
+ * after time_budget_s (0 = none).  The decomposed nodes of each side since
+ * the call go to *gpu_decomposed and *cpu_decomposed (may be NULL). */
+int pbb_hybrid_run(pbb_engine *e, pbb_cpu *c, double period_s, double time_budget_s, uint64_t *gpu_decomposed,
+                   uint64_t *cpu_decomposed, int *all_done);
+
 /* ------------------------------------------------ work-unit codec
  * Factoradic digit vectors (n digits, digit i in [0, n-1-i], the fact_work
  * pos/end layout) <-> decimal interval ranks sum_i digit_i * (n-1-i)!

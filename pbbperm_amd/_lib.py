@@ -145,6 +145,18 @@ SIGNATURES = {
     "pbb_pool_batch_histograms": (C.c_int, [VP, C.c_int, U64P, U64P]),
     "pbb_pool_autotune": (C.c_int, [C.POINTER(Config), C.POINTER(PoolParams), C.c_int, I32P, IP, IP, IP,
                                     C.POINTER(PoolParams), C.POINTER(C.c_double), C.c_int, IP]),
+    "pbb_cpu_create": (C.c_int, [I32P, C.c_int, C.c_int, C.POINTER(Config), C.c_int, C.c_int, IP, C.POINTER(VP)]),
+    "pbb_cpu_destroy": (C.c_int, [VP]),
+    "pbb_cpu_set_root": (C.c_int, [VP, IP]),
+    "pbb_cpu_init_intervals": (C.c_int, [VP, C.c_int, IP, IP, IP]),
+    "pbb_cpu_init_full": (C.c_int, [VP]),
+    "pbb_cpu_run": (C.c_int, [VP, C.c_uint64, C.c_double, IP]),
+    "pbb_cpu_get_intervals": (C.c_int, [VP, IP, IP, IP, IP, C.c_int, U64P, U64P]),
+    "pbb_cpu_get_incumbent": (C.c_int, [VP, IP, IP]),
+    "pbb_cpu_set_incumbent": (C.c_int, [VP, C.c_int, IP]),
+    "pbb_cpu_get_stats": (C.c_int, [VP, C.POINTER(Stats)]),
+    "pbb_cpu_interrupt": (C.c_int, [VP]),
+    "pbb_hybrid_run": (C.c_int, [VP, VP, C.c_double, C.c_double, U64P, U64P, IP]),
     "pbb_fact_to_dec": (C.c_int, [C.c_int, IP, C.c_char_p, C.c_int]),
     "pbb_dec_to_fact": (C.c_int, [C.c_int, C.c_char_p, IP]),
     "pbb_checkpoint_format": (C.c_int, [C.POINTER(Checkpoint), IP, IP, IP, C.c_char_p, C.c_int, IP]),

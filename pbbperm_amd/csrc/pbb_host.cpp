@@ -291,7 +291,7 @@ extern "C" int pbb_instance_read_vrf(const char *path, int *n_out, int *m_out, i
 }
 
 // every job id 0..n-1 exactly once (the ids index the time matrix)
-static bool is_perm(const int *p, int n)
+bool pbb::is_perm(const int *p, int n)
 {
     std::vector<char> seen(n, 0);
     for (int i = 0; i < n; i++)
@@ -356,13 +356,7 @@ extern "C" int pbb_neh(const int32_t *ptm, int n, int m, int *perm, int *cost)
 }
 
 // ------------------------------------------------------------ device tables
-struct HostTables {
-    int n, m, mp, npairs;
-    std::vector<int> ptm, ptmT, min_arr, min_dep, pair_ab;
-    std::vector<int4> jtab;
-};
-
-static void build_tables(const int32_t *ptm, int n, int m, HostTables &T)
+void pbb::build_tables(const int32_t *ptm, int n, int m, HostTables &T)
 {
     T.n = n;
     T.m = m;

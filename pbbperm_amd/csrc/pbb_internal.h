@@ -6,7 +6,23 @@
 
 #include "../../include/pbb_engine.h"
 
+#include <vector>
+
 namespace pbb {
+
+// Host copy of one instance's bound tables (bound_fsp_weak::init,
+// fillMinTimeArrDep, bound_fsp_strong::remplirMachine/remplirLag/
+// remplirTabJohnson): machine-major and job-major times, min heads/tails,
+// the machine pairs a | b << 8 (all m(m-1)/2, lexicographic) and per pair the
+// Johnson order {job, p_a, p_b, lag}.
+struct HostTables {
+    int n, m, mp, npairs;
+    std::vector<int> ptm, ptmT, min_arr, min_dep, pair_ab;
+    std::vector<int4> jtab;
+};
+void build_tables(const int32_t *ptm, int n, int m, HostTables &T);
+// every job id 0..n-1 exactly once
+bool is_perm(const int *p, int n);
 
 // record a message for pbb_last_error() and return code
 int fail(int code, const char *fmt, ...);

@@ -319,6 +319,106 @@ class GpuBB:
         return self.next(node_budget, time_budget)
 
 
+class CpuBB:
+    """Host explorers on the engine's work units (the multicore worker of
+    distributed/src/worker_mc.cpp): same interval, incumbent and statistics
+    calls as GpuBB."""
+
+    def __init__(self, ptm: np.ndarray, config: Optional[Config] = None, threads: int = 0, ub: int = INT_MAX,
+                 ub_perm: Optional[Sequence[int]] = None, root: Optional[Sequence[int]] = None):
+        ptm = np.ascontiguousarray(ptm, dtype=np.int32)
+        self.m, self.n = ptm.shape
+        self.config = config if config is not None else default_config()
+        up = None if ub_perm is None else _lib.sized(ub_perm, np.intc, self.n, "ub_perm")
+        r = None if root is None else _lib.sized(root, np.intc, self.n, "root order")
+        self._h = C.c_void_p()
+        check(load().pbb_cpu_create(_i32(ptm), self.n, self.m, C.byref(self.config), int(threads), int(ub),
+                                    None if up is None else _ip(up), C.byref(self._h)))
+        if r is not None:
+            check(load().pbb_cpu_set_root(self._h, _ip(r)))
+
+    def close(self):
+        if self._h:
+            load().pbb_cpu_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def init_full_interval(self):
+        check(load().pbb_cpu_init_full(self._h))
+
+    def init_from_fac(self, ids: Sequence[int], pos: np.ndarray, end: np.ndarray):
+        ids = np.ascontiguousarray(ids, np.intc)
+        pos = np.ascontiguousarray(pos, np.intc).reshape(-1)
+        end = np.ascontiguousarray(end, np.intc).reshape(-1)
+        if pos.size != len(ids) * self.n or end.size != len(ids) * self.n:
+            raise ValueError("pos/end must hold nb*n factoradic digits")
+        check(load().pbb_cpu_init_intervals(self._h, len(ids), _ip(ids), _ip(pos), _ip(end)))
+
+    def next(self, node_budget: int = 0, time_budget: float = DEFAULT_TIME_BUDGET_S) -> bool:
+        done = C.c_int(0)
+        check(load().pbb_cpu_run(self._h, int(node_budget), float(time_budget), C.byref(done)))
+        return bool(done.value)
+
+    def get_intervals(self):
+        T = self.stats().explorers
+        pos = np.zeros(T * self.n, np.intc)
+        end = np.zeros(T * self.n, np.intc)
+        ids = np.zeros(T, np.intc)
+        nb = C.c_int()
+        dec = C.c_uint64()
+        lv = C.c_uint64()
+        check(load().pbb_cpu_get_intervals(self._h, _ip(pos), _ip(end), _ip(ids), C.byref(nb), T, C.byref(dec),
+                                           C.byref(lv)))
+        k = nb.value
+        return (ids[:k].copy(), pos[: k * self.n].reshape(k, self.n).copy(),
+                end[: k * self.n].reshape(k, self.n).copy(), dec.value, lv.value)
+
+    def interrupt_exploration(self):
+        check(load().pbb_cpu_interrupt(self._h))
+
+    def best(self) -> Tuple[int, np.ndarray]:
+        cost = C.c_int()
+        perm = np.zeros(self.n, np.intc)
+        check(load().pbb_cpu_get_incumbent(self._h, C.byref(cost), _ip(perm)))
+        return cost.value, perm
+
+    def set_best(self, cost: int, perm: Optional[Sequence[int]] = None):
+        p = None if perm is None else _lib.sized(perm, np.intc, self.n, "incumbent permutation")
+        check(load().pbb_cpu_set_incumbent(self._h, int(cost), None if p is None else _ip(p)))
+
+    def stats(self) -> SolveStats:
+        st = Stats()
+        check(load().pbb_cpu_get_stats(self._h, C.byref(st)))
+        return SolveStats(st.decomposed, st.leaves, st.simple_bounds, st.johnson_bounds, st.explorers,
+                          st.active, st.kernel_ms, list(st.depth_histo)[: self.n + 1])
+
+    def solve(self, node_budget: int = 0, time_budget: float = DEFAULT_TIME_BUDGET_S) -> bool:
+        self.init_full_interval()
+        return self.next(node_budget, time_budget)
+
+
+def hybrid_run(gpu: GpuBB, cpu: CpuBB, period: float = 0.05, time_budget: float = 0.0):
+    """The engine and the host explorers on one search (pbb_hybrid_run):
+    returns (all_done, gpu_decomposed, cpu_decomposed)."""
+    g = C.c_uint64()
+    c = C.c_uint64()
+    done = C.c_int()
+    check(load().pbb_hybrid_run(gpu._h, cpu._h, float(period), float(time_budget), C.byref(g), C.byref(c),
+                                C.byref(done)))
+    return bool(done.value), g.value, c.value
+
+
 # ------------------------------------------------------- work-unit codec
 
 def fact_to_dec(digits: Sequence[int]) -> str:

@@ -442,3 +442,56 @@ def test_engine_rejects_bad_config():
     with pytest.raises(P.PbbError):
         eng.next()
     eng.close()
+
+
+def test_host_coworker_shares_one_search_with_the_engine(oracle_mod):
+    """The engine and the host co-worker on one search with the incumbent
+    frozen at the optimum: after a budgeted engine run, its exported
+    intervals are split between the host threads and the engine, both run
+    to the end at the same time, and the counts of both sides add up to the
+    engine's first segment plus the oracle's counts of every exported
+    interval (sequentialbb::initAtInterval semantics on each).  Then
+    pbb_hybrid_run from the root with an improving incumbent finds the
+    optimum and a schedule realising it."""
+    import threading
+    import pbbperm_amd as P
+    O = oracle_mod
+    for (seed, n, m, mode) in [(42, 16, 8, (0, 1, 0, 0)), (3, 16, 6, (2, 3, 1, 0))]:
+        bm, br, rf, jp = mode
+        ptm = O.synth_ptm(seed, n, m)
+        opt = _opt(O, ptm)
+        print(f"hybrid seed={seed} mode={mode}", flush=True)
+        cfg = _cfg(P, bm, br, rf, jp, nb=512)
+        eng = P.GpuBB(ptm, cfg, ub=opt)
+        eng.init_full_interval()
+        assert not eng.next(node_budget=20000, time_budget=60.0)
+        ids, pos, end, dec1, _ = eng.get_intervals()
+        assert len(ids) >= 4
+        want = dec1 + sum(O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp,
+                                  pos=p, end=e).decomposed for p, e in zip(pos, end))
+        h = min(8, len(ids) // 2)
+        cpu = P.CpuBB(ptm, cfg, threads=8, ub=opt)
+        cpu.init_from_fac(np.arange(h), pos[:h], end[:h])
+        eng.init_from_fac(np.arange(len(ids) - h), pos[h:], end[h:])
+        res = {}
+        t = threading.Thread(target=lambda: res.update(cpu=cpu.next(time_budget=120.0)))
+        t.start()
+        assert eng.next(time_budget=60.0)
+        t.join()
+        assert res["cpu"]
+        assert eng.stats().decomposed + cpu.stats().decomposed == want
+        eng.close()
+        cpu.close()
+    ptm = O.synth_ptm(42, 16, 8)
+    opt = _opt(O, ptm)
+    nehc, nehp = O.Instance(ptm).neh()
+    cfg = _cfg(P, 0, 1, 0, 0, nb=1024)
+    with P.GpuBB(ptm, cfg, ub=nehc, ub_perm=nehp) as eng, P.CpuBB(ptm, cfg, threads=8, ub=nehc, ub_perm=nehp) as cpu:
+        eng.init_full_interval()
+        done, gdec, cdec = P.hybrid_run(eng, cpu, period=0.002, time_budget=120.0)
+        assert done and gdec > 0
+        gb, gp = eng.best()
+        cb, cp = cpu.best()
+        assert min(gb, cb) == opt
+        best_perm = gp if gb <= cb else cp
+        assert O.Instance(ptm).makespan(best_perm) == opt
