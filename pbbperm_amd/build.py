@@ -44,7 +44,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         for src in ("pbb_kernels", "pbb_pool"):
             obj = os.path.join(BUILD, f"{src}_m{m}.o")
             units.append([HIPCC, *FLAGS, f"-DPBB_M={m}", "-c", os.path.join(CSRC, src + ".hip"), "-o", obj])
-    for src in ("pbb_dispatch.cpp", "pbb_host.cpp", "pbb_pool.cpp", "pbb_checkpoint.cpp", "pbb_cpu.cpp"):
+    for src in ("pbb_dispatch.cpp", "pbb_host.cpp", "pbb_io.cpp", "pbb_pool.cpp", "pbb_checkpoint.cpp", "pbb_cpu.cpp"):
         obj = os.path.join(BUILD, src.replace(".cpp", ".o"))
         units.append([HIPCC, *FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj])
     # compiled once: it defines the one device array of the allocation-free entry point

@@ -17,7 +17,7 @@
 #include <vector>
 
 #include "../../include/pbb_engine.h"
-#include "pbb_internal.h"
+#include "pbb_util.h"
 
 namespace {
 

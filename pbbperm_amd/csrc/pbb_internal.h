@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/pbb_engine.h"
+#include "pbb_util.h"
 
 #include <vector>
 
@@ -21,17 +22,7 @@ struct HostTables {
     std::vector<int4> jtab;
 };
 void build_tables(const int32_t *ptm, int n, int m, HostTables &T);
-// every job id 0..n-1 exactly once
-bool is_perm(const int *p, int n);
-
-// record a message for pbb_last_error() and return code
-int fail(int code, const char *fmt, ...);
-int validate(const pbb_config *c);
-int check_shape(const int32_t *ptm, int n, int m);
 int check_device();
-// Stream replays report on stderr every 500 served requests when the
-// environment sets PBB_REPLAY_PROGRESS (a long stream runs for minutes).
-void replay_progress(const char *what, int before, int after, int count);
 
 }  // namespace pbb
 
