@@ -202,6 +202,44 @@ def test_serving_results_alone_equal_results_in_the_mix():
           f"overlapping admissions {overlap}); plain {pwall:.3f} s")
 
 
+def test_serving_wide_stream_matches_reference():
+    """The serving stream's whole shape range (n in [12,30], m in
+    {5,10,20}), stratified by machine count and job band (tests/golden/
+    serve_wide.json: the first requests of the stream, seeds 10000+, of each
+    stratum whose search the C oracle finishes within 3 M nodes; 40
+    requests, none at m = 20 with n >= 21, where none of the 40 probed per
+    band finished).  Served through one 20-machine pool of up to 30 jobs as
+    a Poisson stream: from the NEH incumbent every optimum is the
+    reference's and the schedule realises it; with the incumbent frozen at
+    the optimum every node count is the reference's; the plain
+    one-instance-per-launch path gives the same counts."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    with open(os.path.join(HERE, "golden", "serve_wide.json")) as f:
+        fx = json.load(f)
+    sp = fx["spec"]
+    inst = fx["instances"]
+    assert max(d["n"] for d in inst) >= 29 and {d["m"] for d in inst} == {5, 10, 20}
+    ptms = [P.synth(d["seed"], d["n"], d["m"]) for d in inst]
+    cfg = _cfg(P, sp["bound_mode"], sp["branching"], sp["refine"], sp["pairs"])
+    arrivals = PP.poisson_arrivals(len(ptms), rate=200.0, seed=4)
+    with PP.Pool(cfg, max_jobs=30, machines=20, slots=16, waves_per_group=4, explorers_per_instance=512) as pool:
+        imp, _, _ = pool.replay(ptms, [d["neh"] for d in inst], arrivals, time_budget=120.0)
+        frz, lat, wall = pool.replay(ptms, [d["opt"] for d in inst], arrivals, time_budget=120.0)
+    for p, r, d in zip(ptms, imp, inst):
+        assert r.status == "solved" and r.best == d["opt"], d
+        _check_schedule(P, p, r)
+    for r, d in zip(frz, inst):
+        assert r.status == "solved" and _counts(r) == d["counts_frozen"], d
+    plain, _, _ = PP.plain_replay(ptms, [d["opt"] for d in inst], arrivals, config=P.default_config(
+        bound_mode=sp["bound_mode"], branching=sp["branching"], refine=sp["refine"],
+        johnson_pairs=sp["pairs"], early_stop_johnson=0, nb_ivms=4096), time_budget=120.0)
+    for r, d in zip(plain, inst):
+        assert r.status == "solved" and _counts(r) == d["counts_frozen"], d
+    print(f"wide stream: {len(inst)} requests, {sum(d['counts_frozen'][0] for d in inst)} frozen-bound nodes, "
+          f"served in {wall:.2f} s (p99 {np.percentile(lat, 99):.1f} ms)")
+
+
 def test_serving_improving_ub_optimum_exact():
     import pbbperm_amd as P
     import pbbperm_amd.pool as PP
