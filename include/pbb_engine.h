@@ -62,6 +62,15 @@ typedef struct pbb_config {
     int checkpoint_s;       /* [time] checkpoint */
     int balance_s;          /* [time] balance: pbb_run returns after this period
                                in worker mode */
+    int ramp_nodes;         /* an explorer donates to idle explorers only after
+                               decomposing this many nodes of its current
+                               interval (0: at once); engine-specific, no
+                               reference counterpart */
+    int team_nodes;         /* a search from pbb_init_full first runs on one
+                               workgroup of explorers (16 with LB1 only, 8 with
+                               a refinement bound) for up to this many nodes,
+                               then on the whole device (0: whole device at
+                               once); engine-specific, no reference counterpart */
 } pbb_config;
 
 /* Defaults of arguments.cpp:135-175 with sortedDFS forced to 0. */

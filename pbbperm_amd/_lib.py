@@ -35,7 +35,8 @@ class Config(C.Structure):
         ("johnson_pairs", C.c_int), ("early_stop_johnson", C.c_int), ("find_all", C.c_int),
         ("single_node", C.c_int), ("sorted_dfs", C.c_int), ("nb_ivms", C.c_int),
         ("steal_margin", C.c_int), ("init_mode", C.c_int), ("init_solution", C.c_int),
-        ("checkpoint_s", C.c_int), ("balance_s", C.c_int),
+        ("checkpoint_s", C.c_int), ("balance_s", C.c_int), ("ramp_nodes", C.c_int),
+        ("team_nodes", C.c_int),
     ]
 
     def as_dict(self):

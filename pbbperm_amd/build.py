@@ -50,6 +50,8 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     # compiled once: it defines the one device array of the allocation-free entry point
     units.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, "pbb_noalloc.hip"),
                   "-o", os.path.join(BUILD, "pbb_noalloc.o")])
+    units.append([HIPCC, *FLAGS, "-c", os.path.join(CSRC, "pbb_setup.hip"),
+                  "-o", os.path.join(BUILD, "pbb_setup.o")])
     with cf.ThreadPoolExecutor(jobs) as ex:
         errs = [e for e in ex.map(_run, units) if e]
     if errs:

@@ -35,6 +35,13 @@ hipError_t launch_explore(const KArgs &a, int blocks, hipStream_t s)
 #undef C_
 }
 
+hipError_t launch_team(const KArgs &a, hipStream_t s)
+{
+#define C_(M) launch_team_m<M>(a, s)
+    PBB_FOR_M(C_, a.I.m)
+#undef C_
+}
+
 hipError_t explore_occupancy(const KArgs &a, int *blocks_per_cu)
 {
 #define C_(M) occupancy_m<M>(a, blocks_per_cu)
@@ -42,7 +49,7 @@ hipError_t explore_occupancy(const KArgs &a, int *blocks_per_cu)
 #undef C_
 }
 
-hipError_t launch_root(const KArgs &a, const int *root_perm, hipStream_t s)
+hipError_t launch_root(const KArgs &a, const RootPerm &root_perm, hipStream_t s)
 {
 #define C_(M) launch_root_m<M>(a, root_perm, s)
     PBB_FOR_M(C_, a.I.m)

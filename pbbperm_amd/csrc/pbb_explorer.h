@@ -13,77 +13,120 @@
 namespace pbb {
 
 // ------------------------------------------------------------------ atomics
+// Every helper takes the memory scope of the explorers that share the
+// words: the agent for a launch over the device, the workgroup for a team
+// of explorers in one workgroup (whose release/acquire then need no L2
+// writeback or invalidate).
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ int ld_acq(const int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, S);
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ int ld_rlx(const int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, S);
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ unsigned ld_rlx(const unsigned *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, S);
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, S);
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ void st_rel(int *p, int v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, S);
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ void st_rlx(int *p, int v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, S);
+}
+
+template <int S>
+__device__ __forceinline__ void fence_acq()
+{
+    if constexpr (S == __HIP_MEMORY_SCOPE_WORKGROUP) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
 // lane-0 read-modify-writes, result broadcast to the wavefront
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ int w_add(int *p, int v)
 {
     int r = 0;
-    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, S);
     return rdlane(r, 0);
 }
-__device__ __forceinline__ unsigned w_addu(unsigned *p, unsigned v)
+// Relaxed counters: on this multi-XCD device an agent-scope release writes
+// back the wave's L2 and an acquire invalidates it, so the idle-ring tickets
+// and the work counter, whose values carry no other data, take no fence.
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
+__device__ __forceinline__ int w_add_rlx(int *p, int v)
+{
+    int r = 0;
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, S);
+    return rdlane(r, 0);
+}
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
+__device__ __forceinline__ unsigned w_addu_rlx(unsigned *p, unsigned v)
 {
     unsigned r = 0;
-    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, S);
     return (unsigned)rdlane((int)r, 0);
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
+__device__ __forceinline__ unsigned long long ld_rlx64(const unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, S);
+}
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
+__device__ __forceinline__ void st_rlx64(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, S);
+}
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ unsigned long long w_add64(unsigned long long *p, unsigned long long v)
 {
     unsigned long long r = 0;
-    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane_id() == 0) r = __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, S);
     unsigned lo = (unsigned)rdlane((int)(unsigned)r, 0);
     unsigned hi = (unsigned)rdlane((int)(unsigned)(r >> 32), 0);
     return ((unsigned long long)hi << 32) | lo;
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ bool w_cas(unsigned *p, unsigned expect, unsigned desired)
 {
     int ok = 0;
     if (lane_id() == 0) {
         unsigned e = expect;
         ok = __hip_atomic_compare_exchange_strong(p, &e, desired, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
+                                                  S);
     }
     return rdlane(ok, 0) != 0;
 }
 // Relaxed claim: no agent-scope release/acquire, so a failed attempt costs
 // one atomic round trip and no L2 writeback or invalidate.
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ bool w_cas_rlx(unsigned *p, unsigned expect, unsigned desired)
 {
     int ok = 0;
     if (lane_id() == 0) {
         unsigned e = expect;
         ok = __hip_atomic_compare_exchange_strong(p, &e, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
+                                                  S);
     }
     return rdlane(ok, 0) != 0;
 }
+template <int S = __HIP_MEMORY_SCOPE_AGENT>
 __device__ __forceinline__ void w_min64(unsigned long long *p, unsigned long long v)
 {
-    if (lane_id() == 0) __hip_atomic_fetch_min(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (lane_id() == 0) __hip_atomic_fetch_min(p, v, __ATOMIC_ACQ_REL, S);
 }
 
 // ------------------------------------------------------ lane scans (DPP)
@@ -238,8 +281,7 @@ __device__ __forceinline__ InstRef engine_inst(const KArgs &a)
 {
     InstRef r;
     r.ctl = a.ctl;
-    r.ring_seq = a.ring_seq;
-    r.ring_id = a.ring_id;
+    r.ring = a.ring;
     r.ring_cap = a.ring_cap;
     r.ub0 = a.initial_ub;
     r.best = a.best;
@@ -262,7 +304,7 @@ __device__ __forceinline__ InstRef engine_inst(const KArgs &a)
 // its tail).
 // REF: 0 LB1 only (boundingMode 0); 1 LB1 + LB1' refinement (boundingMode
 // 1/2 as shipped); 2 LB1 + Johnson LB2 refinement.
-template <int M, int REF>
+template <int M, int REF, int SC = __HIP_MEMORY_SCOPE_AGENT>
 struct Explorer {
     const KArgs &a;
     Lds lds;
@@ -279,6 +321,7 @@ struct Explorer {
     int local_best;
     unsigned dec, leaves, sb, jb;     // deltas since the last flush
     unsigned nchk;                    // periodic checks so far (host-word phase)
+    unsigned since;                   // nodes decomposed in this launch's own interval (ramp)
     bool end_dirty;                   // end moved since pos was last compared with it
     int hist;                         // lane d: nodes decomposed at depth d since the flush
     int jhist;                        // lane d: refinement bounds at depth d (REF != 0)
@@ -294,6 +337,7 @@ struct Explorer {
         ftop = 0;
         if constexpr (REF == 2) pair_lanes<PCH>(r.pair_ab, r.npairs, r.m, r.pairs_mode, pinfo);
         dec = leaves = sb = jb = 0;
+        since = 0;
         nchk = (unsigned)eid;  // staggers the host-word reads across explorers
         end_dirty = true;
         hist = jhist = 0;
@@ -730,7 +774,7 @@ struct Explorer {
         if (!gt_best(cost)) {
             if (cost < local_best) {
                 write_perm(r);
-                w_min64(ir.best, ((unsigned long long)(unsigned)cost << 32) | (unsigned)e);
+                w_min64<SC>(ir.best, ((unsigned long long)(unsigned)cost << 32) | (unsigned)e);
             }
             local_best = cost;
         }
@@ -837,6 +881,7 @@ struct Explorer {
                 continue;
             }
             dec++;
+            since += since < a.ramp ? 1u : 0u;  // saturates at the ramp
             go_down(c, true);
             hist += (lane == L);
             if (L == n - 1) {
@@ -870,6 +915,7 @@ struct Explorer {
         endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
         dirv = lane < n ? (int)a.dir[(size_t)e * 64 + lane] : 0;
         end_dirty = true;
+        since = a.ramp;  // a resumed or donated interval is shared at once
         __builtin_amdgcn_wave_barrier();
         rebuild_stacks();
     }
@@ -886,7 +932,7 @@ struct Explorer {
             a.dir[(size_t)e * 64 + lane] = (int8_t)dirv;
         }
         if (lane == 0) a.line[e] = L;
-        if (lane == 0) st_rlx(a.state + e, st);
+        if (lane == 0) st_rlx<SC>(a.state + e, st);
     }
 
     // fresh interval: row 0 = bounded root row, pos/end from HBM
@@ -898,6 +944,7 @@ struct Explorer {
         endv = lane < n ? (int)a.end[(size_t)e * 64 + lane] : 0;
         dirv = lane == 0 ? ir.root_dir[0] : 0;
         end_dirty = true;
+        since = 0;
         reset_stacks();
         if (!before_end()) return false;
         unfold();
@@ -914,6 +961,7 @@ struct Explorer {
         endv = lane < n ? n - 1 - lane : 0;
         dirv = lane == 0 ? dirv : 0;
         end_dirty = true;
+        since = 0;
         reset_stacks();
         if (!before_end()) return false;
         unfold();
@@ -933,8 +981,8 @@ struct Explorer {
     __device__ __forceinline__ void fail(int code)
     {
         if (lane == 0) {
-            st_rlx(&run->err, code);
-            st_rlx(&run->stop, 1);
+            st_rlx<SC>(&run->err, code);
+            st_rlx<SC>(&run->stop, 1);
         }
     }
 
@@ -947,6 +995,11 @@ struct Explorer {
     // its end is trimmed to pos (deeper ends become the maximum, so the
     // victim's interval covers exactly the same live nodes) and the split
     // moves down to the next line.  Node counts are unchanged by either step.
+    // An explorer that starts a launch on a fresh interval (the root
+    // interval, imported intervals) decomposes `ramp` nodes before it hands
+    // work on: a small tree then runs depth-first in one explorer, finding
+    // the incumbents the reference's order finds, and a large one fans out
+    // after that many nodes (woken and resumed explorers share at once).
     // h, t: idle-ring head/tail loaded before the node's work (prefetch).
     // Returns whether idle explorers were waiting (the ring was not empty).
     __device__ __forceinline__ bool try_donate(unsigned h, unsigned t)
@@ -985,12 +1038,16 @@ struct Explorer {
         // the thief's id), so it is relaxed: with thousands of contenders an
         // acq_rel attempt, i.e. an L2 writeback + invalidate per loser,
         // dominated the ramp-up of every solve.
-        if (!w_cas_rlx(&ir.ctl->idle_head, h, h + 1)) return true;
-        // claim the idle explorer registered at ticket h (bounded MPMC ring cell)
+        if (!w_cas_rlx<SC>(&ir.ctl->idle_head, h, h + 1)) return true;
+        // claim the idle explorer registered at ticket h (bounded MPMC ring
+        // cell: the ticket and the explorer id share one word, so reading
+        // the id needs no acquire)
         const unsigned slot = h & (unsigned)(ir.ring_cap - 1);
         bool ok = false;
+        unsigned long long cell = 0ull;
         for (int spin = 0; spin < PBB_SPIN_LIMIT; spin++) {
-            if ((unsigned)uni(ld_acq((const int *)(ir.ring_seq + slot))) == h + 1u) {
+            cell = uni64(ld_rlx64<SC>(ir.ring + slot));
+            if ((unsigned)(cell >> 32) == h + 1u) {
                 ok = true;
                 break;
             }
@@ -1000,8 +1057,9 @@ struct Explorer {
             fail(PBB_DEV_ERR_RING);
             return true;
         }
-        const int thief = uni(ld_rlx(ir.ring_id + slot)) - 1;
-        if (lane == 0) st_rel((int *)(ir.ring_seq + slot), (int)(h + (unsigned)ir.ring_cap));
+        const int thief = (int)(unsigned)cell - 1;
+        // the cell waits for the ticket of the next lap
+        if (lane == 0) st_rlx64<SC>(ir.ring + slot, (unsigned long long)(h + (unsigned)ir.ring_cap) << 32);
         if (thief < 0 || thief >= a.E) {
             fail(PBB_DEV_ERR_RING);
             return true;
@@ -1030,20 +1088,26 @@ struct Explorer {
         if (lane == l) endv = cut - 1;
         if (lane > l && lane < n) endv = n - 1 - lane;
         end_dirty = true;
-        w_add(&ir.ctl->work_count, 1);
-        if (lane == 0) st_rel(a.state + thief, ST_ACTIVE);
+        // The thief's count is added before the release below publishes its
+        // interval, so its own decrement (after its acquire) comes later in
+        // the counter's order: relaxed suffices.
+        w_add_rlx<SC>(&ir.ctl->work_count, 1);
+        if (lane == 0) st_rel<SC>(a.state + thief, ST_ACTIVE);
         return true;
     }
 
-    // enter the idle ring (cell sequence protocol of a bounded MPMC queue)
+    // enter the idle ring (cell sequence protocol of a bounded MPMC queue:
+    // ticket t owns cell t mod cap once the cell shows t; storing t + 1 with
+    // this explorer's id hands it to the consumer of ticket t).  All relaxed:
+    // the cell word carries everything the consumer reads.
     __device__ __forceinline__ void register_idle()
     {
-        if (lane == 0) st_rlx(a.state + e, ST_IDLE);
-        unsigned t = w_addu(&ir.ctl->idle_tail, 1u);
+        if (lane == 0) st_rlx<SC>(a.state + e, ST_IDLE);
+        unsigned t = w_addu_rlx<SC>(&ir.ctl->idle_tail, 1u);
         const unsigned slot = t & (unsigned)(ir.ring_cap - 1);
         bool ok = false;
         for (int spin = 0; spin < PBB_SPIN_LIMIT; spin++) {
-            if ((unsigned)uni(ld_acq((const int *)(ir.ring_seq + slot))) == t) {
+            if ((unsigned)(uni64(ld_rlx64<SC>(ir.ring + slot)) >> 32) == t) {
                 ok = true;
                 break;
             }
@@ -1051,9 +1115,8 @@ struct Explorer {
         }
         if (!ok) {
             fail(PBB_DEV_ERR_RING);
-        } else {
-            if (lane == 0) st_rlx(ir.ring_id + slot, e + 1);
-            if (lane == 0) st_rel((int *)(ir.ring_seq + slot), (int)(t + 1u));
+        } else if (lane == 0) {
+            st_rlx64<SC>(ir.ring + slot, ((unsigned long long)(t + 1u) << 32) | (unsigned)(e + 1));
         }
     }
 
@@ -1062,7 +1125,7 @@ struct Explorer {
     __device__ __forceinline__ bool go_idle()
     {
         register_idle();
-        int left = w_add(&ir.ctl->work_count, -1);
+        int left = w_add_rlx<SC>(&ir.ctl->work_count, -1);
         return left == 1;
     }
 
@@ -1094,14 +1157,14 @@ struct Explorer {
     __device__ __forceinline__ bool check_stop()
     {
         if (a.node_budget) {
-            unsigned long long before = w_add64(&ir.ctl->dec, (unsigned long long)dec);
-            if (before + dec >= a.node_budget && lane == 0) st_rlx(&run->stop, 1);
+            unsigned long long before = w_add64<SC>(&ir.ctl->dec, (unsigned long long)dec);
+            if (before + dec >= a.node_budget && lane == 0) st_rlx<SC>(&run->stop, 1);
             dec = 0;
         }
         if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
-        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(&run->stop, 1);
+        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx<SC>(&run->stop, 1);
         poll_host_and_best();
-        return uni(ld_rlx(&run->stop)) != 0;
+        return uni(ld_rlx<SC>(&run->stop)) != 0;
     }
 
     // The pool's periodic check: the deadline (the launch's or the request's,
@@ -1111,10 +1174,10 @@ struct Explorer {
     __device__ __forceinline__ int check_stop_slot(int *slot_stop)
     {
         if (dec >= (1u << 28) || sb >= (1u << 28) || jb >= (1u << 28)) flush_counters();  // 32-bit deltas
-        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx(slot_stop, 1);
+        if (t_end && wall_clock64() >= t_end && lane == 0) st_rlx<SC>(slot_stop, 1);
         poll_host_and_best();
-        if (uni(ld_rlx(&run->stop)) != 0) return 1;
-        return uni(ld_rlx(slot_stop)) != 0 ? 2 : 0;
+        if (uni(ld_rlx<SC>(&run->stop)) != 0) return 1;
+        return uni(ld_rlx<SC>(slot_stop)) != 0 ? 2 : 0;
     }
 
     __device__ __forceinline__ void poll_host_and_best()
@@ -1122,9 +1185,9 @@ struct Explorer {
         const bool host_turn = (nchk++ & (PBB_HOST_POLL_CHECKS - 1u)) == 0u;
         if (a.host_flag && host_turn && lane == 0) {
             int hf = __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            if (hf) st_rlx(&run->stop, 1);
+            if (hf) st_rlx<SC>(&run->stop, 1);
         }
-        unsigned long long b = ld_rlx(ir.best);
+        unsigned long long b = ld_rlx<SC>(ir.best);
         int bc = uni((int)(unsigned)(b >> 32));
         if (bc < local_best) local_best = bc;
     }

@@ -217,12 +217,15 @@ struct pbb_engine {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     // device buffers
     int8_t *mat = nullptr, *pos = nullptr, *end = nullptr, *dir = nullptr;
-    int *line = nullptr, *state = nullptr, *ring_id = nullptr;
-    unsigned *ring_seq = nullptr;
+    int *line = nullptr, *state = nullptr;
+    unsigned long long *ring = nullptr;
     Ctl *ctl = nullptr;
     unsigned long long *best = nullptr;
-    int *best_perm = nullptr, *root_row = nullptr, *root_dir = nullptr, *root_perm_d = nullptr;
+    int *best_perm = nullptr, *root_row = nullptr, *root_dir = nullptr;
     int *host_flag = nullptr, *host_flag_d = nullptr;
+    // pinned read-back of the control block and the incumbent word after a launch
+    Ctl *ctl_h = nullptr;
+    unsigned long long *best_h = nullptr;
     // host state
     std::vector<int> root_perm;
     int inc_cost;
@@ -234,7 +237,14 @@ struct pbb_engine {
     // counters carried in from a checkpoint file (works::init_infile restores
     // simpleBounds, distributed/src/works.cpp:81-83; the unit's exploredNodes)
     unsigned long long dec_base = 0, sb_base = 0;
-    std::vector<unsigned> ring_seq0;  // the empty ring's cell sequence numbers
+    // the device control block's done flag and node count as of the last
+    // set-up or launch (nothing else writes them between launches)
+    bool done_host = false;
+    unsigned long long dec_dev = 0;
+    // the team launch: its explorer count (0: none fits) and whether the
+    // next pbb_run starts with it (set by pbb_init_full)
+    int team_E = 0;
+    bool team_pending = false;
     double last_ms = 0.0;
     int wall_khz = 100000;
 };
@@ -251,8 +261,7 @@ static KArgs make_args(pbb_engine *e)
     a.dir = e->dir;
     a.line = e->line;
     a.state = e->state;
-    a.ring_id = e->ring_id;
-    a.ring_seq = e->ring_seq;
+    a.ring = e->ring;
     a.ring_cap = e->ring_cap;
     a.ctl = e->ctl;
     a.best = e->best;
@@ -270,7 +279,17 @@ static KArgs make_args(pbb_engine *e)
     a.single_node = e->cfg.single_node;
     a.initial_ub = e->initial_ub;
     a.steal_margin = e->cfg.steal_margin;
+    a.ramp = (unsigned)e->cfg.ramp_nodes;
     return a;
+}
+
+// Every copy of engine state is ordered on the engine's stream (launches are
+// not followed by a wait of their own; the host waits where it reads).
+static hipError_t copy_sync(pbb_engine *e, void *dst, const void *src, size_t bytes, hipMemcpyKind kind)
+{
+    hipError_t r = hipMemcpyAsync(dst, src, bytes, kind, e->stream);
+    if (r == hipSuccess) r = hipStreamSynchronize(e->stream);
+    return r;
 }
 
 static int capacity(pbb_engine *e, int *explorers)
@@ -296,15 +315,15 @@ extern "C" int pbb_destroy(pbb_engine *e)
     hipFree(e->dir);
     hipFree(e->line);
     hipFree(e->state);
-    hipFree(e->ring_id);
-    hipFree(e->ring_seq);
+    hipFree(e->ring);
     hipFree(e->ctl);
     hipFree(e->best);
     hipFree(e->best_perm);
     hipFree(e->root_row);
     hipFree(e->root_dir);
-    hipFree(e->root_perm_d);
     if (e->host_flag) hipHostFree(e->host_flag);
+    if (e->ctl_h) hipHostFree(e->ctl_h);
+    if (e->best_h) hipHostFree(e->best_h);
     if (e->ev0) hipEventDestroy(e->ev0);
     if (e->ev1) hipEventDestroy(e->ev1);
     if (e->stream) hipStreamDestroy(e->stream);
@@ -354,6 +373,14 @@ extern "C" int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cf
     TRY(capacity(e, &cap));
     e->E = std::max(1, std::min(cfg->nb_ivms, cap));
     e->blocks = (e->E + PBB_EXPLORE_WAVES - 1) / PBB_EXPLORE_WAVES;
+    {
+        // the team: up to team_waves_max explorers whose LDS fits one workgroup
+        const bool lb2 = cfg->bound_mode != 0 && cfg->refine == 1;
+        const int ref = cfg->bound_mode == 0 ? 0 : (cfg->refine == 1 ? 2 : 1);
+        int w = std::min(e->E, team_waves_max(ref));
+        while (w > 0 && team_lds_bytes(e->D.I, lb2, w) > 160 * 1024) w--;
+        e->team_E = w;
+    }
     e->ring_cap = 1;
     while (e->ring_cap < 2 * e->E) e->ring_cap <<= 1;
     size_t E = (size_t)e->E;
@@ -363,23 +390,23 @@ extern "C" int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cf
     HTRY(hipMalloc(&e->dir, E * 64));
     HTRY(hipMalloc(&e->line, E * 4));
     HTRY(hipMalloc(&e->state, E * 4));
-    HTRY(hipMalloc(&e->ring_id, (size_t)e->ring_cap * 4));
-    HTRY(hipMalloc(&e->ring_seq, (size_t)e->ring_cap * 4));
+    HTRY(hipMalloc(&e->ring, (size_t)e->ring_cap * 8));
     HTRY(hipMalloc(&e->ctl, sizeof(Ctl)));
     HTRY(hipMalloc(&e->best, 8));
     HTRY(hipMalloc(&e->best_perm, E * 64 * 4));
     HTRY(hipMalloc(&e->root_row, 64 * 4));
     HTRY(hipMalloc(&e->root_dir, 4));
-    HTRY(hipMalloc(&e->root_perm_d, 64 * 4));
     HTRY(hipHostMalloc((void **)&e->host_flag, 4, hipHostMallocMapped));
     *e->host_flag = 0;
     HTRY(hipHostGetDevicePointer((void **)&e->host_flag_d, e->host_flag, 0));
+    HTRY(hipHostMalloc((void **)&e->ctl_h, sizeof(Ctl), hipHostMallocDefault));
+    HTRY(hipHostMalloc((void **)&e->best_h, sizeof(unsigned long long), hipHostMallocDefault));
     int dev = 0;
     HTRY(hipGetDevice(&dev));
     HTRY(hipDeviceGetAttribute(&e->wall_khz, hipDeviceAttributeWallClockRate, dev));
-    HTRY(hipMemset(e->state, 0, E * 4));
+    HTRY(hipMemsetAsync(e->state, 0, E * 4, e->stream));
     unsigned long long packed = ((unsigned long long)(unsigned)ub << 32) | 0xffffffffull;
-    HTRY(hipMemcpy(e->best, &packed, 8, hipMemcpyHostToDevice));
+    HTRY(copy_sync(e, e->best, &packed, 8, hipMemcpyHostToDevice));
 #undef HTRY
 #undef TRY
     *out = e;
@@ -399,33 +426,39 @@ extern "C" int pbb_set_root(pbb_engine *e, const int *root_perm)
     return PBB_OK;
 }
 
-static int sync_incumbent(pbb_engine *e)
+// take the device incumbent word (read back already) into the host's copy
+static int take_incumbent(pbb_engine *e, unsigned long long packed)
 {
-    unsigned long long packed = 0;
-    HIPCHK(hipMemcpy(&packed, e->best, 8, hipMemcpyDeviceToHost));
     int cost = (int)(packed >> 32);
     unsigned who = (unsigned)(packed & 0xffffffffull);
     if (cost < e->inc_cost && who != 0xffffffffu && (int)who < e->E) {
         std::vector<int> p(64);
-        HIPCHK(hipMemcpy(p.data(), e->best_perm + (size_t)who * 64, 64 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(copy_sync(e, p.data(), e->best_perm + (size_t)who * 64, 64 * 4, hipMemcpyDeviceToHost));
         e->inc_cost = cost;
         e->inc_perm.assign(p.begin(), p.begin() + e->n);
     }
     return PBB_OK;
 }
 
+static int sync_incumbent(pbb_engine *e)
+{
+    HIPCHK(copy_sync(e, e->best_h, e->best, 8, hipMemcpyDeviceToHost));
+    return take_incumbent(e, *e->best_h);
+}
+
+// ivm_bound::boundRoot (ivm_bound.cpp:273-330) under the current incumbent:
+// the host's copy is current here (every launch ends by reading the device
+// word back, and pbb_set_incumbent writes both).  The launch is only
+// enqueued; the set-up and explorer launches after it on the same stream
+// read its output.
 static int bound_root(pbb_engine *e)
 {
     if (e->root_bounded) return PBB_OK;
-    int rc = sync_incumbent(e);
-    if (rc) return rc;
     e->initial_ub = e->inc_cost;
-    std::vector<int> rp(64, 0);
-    std::copy(e->root_perm.begin(), e->root_perm.end(), rp.begin());
-    HIPCHK(hipMemcpy(e->root_perm_d, rp.data(), 64 * 4, hipMemcpyHostToDevice));
+    RootPerm rp{};
+    std::copy(e->root_perm.begin(), e->root_perm.end(), rp.p);
     KArgs a = make_args(e);
-    HIPCHK(launch_root(a, e->root_perm_d, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(launch_root(a, rp, e->stream));
     e->root_bounded = true;
     return PBB_OK;
 }
@@ -451,41 +484,36 @@ static int setup_pool(pbb_engine *e, const std::vector<int> &ids, const int *pos
     memset(&c, 0, sizeof c);
     c.work_count = (int)ids.size();
     c.done = ids.empty() ? 1 : 0;
-    HIPCHK(hipMemcpy(e->state, state.data(), (size_t)E * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->pos, hp.data(), hp.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->end, he.data(), he.size(), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->ctl, &c, sizeof c, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->state, state.data(), (size_t)E * 4, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->pos, hp.data(), hp.size(), hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->end, he.data(), he.size(), hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->ctl, &c, sizeof c, hipMemcpyHostToDevice));
     e->exported_dec = e->exported_leaves = 0;
     e->dec_base = e->sb_base = 0;
+    e->done_host = c.done != 0;
+    e->dec_dev = 0;
+    e->team_pending = false;
     e->initialized = true;
     return PBB_OK;
 }
 
-// Empty the idle ring before a launch: cell i waits for ticket i, both ticket
-// counters at 0.  Explorers idle at the end of the previous launch register
-// again when they run, so the ring never names a wavefront that is not
-// resident.
-static int reset_ring(pbb_engine *e)
-{
-    if (e->ring_seq0.empty()) {
-        e->ring_seq0.resize(e->ring_cap);
-        for (int i = 0; i < e->ring_cap; i++) e->ring_seq0[i] = (unsigned)i;
-    }
-    HIPCHK(hipMemcpy(e->ring_seq, e->ring_seq0.data(), (size_t)e->ring_cap * 4, hipMemcpyHostToDevice));
-    const unsigned zero[2] = {0u, 0u};
-    static_assert(offsetof(Ctl, idle_tail) == offsetof(Ctl, idle_head) + sizeof(unsigned), "ring tickets");
-    HIPCHK(hipMemcpy(&e->ctl->idle_head, zero, sizeof zero, hipMemcpyHostToDevice));
-    return PBB_OK;
-}
-
+// The whole search space to explorer 0, the others idle: two launches
+// enqueued on the engine's stream (root bounding, set-up), no copy and no
+// wait; pbb_run's launches follow them on the stream.
 extern "C" int pbb_init_full(pbb_engine *e)
 {
     if (!e) return fail(PBB_E_ARG, "null engine");
     int rc = bound_root(e);
     if (rc) return rc;
-    std::vector<int> p(e->n, 0), q(e->n);
-    for (int i = 0; i < e->n; i++) q[i] = e->n - i - 1;
-    return setup_pool(e, std::vector<int>{0}, p.data(), q.data());
+    KArgs a = make_args(e);
+    HIPCHK(launch_setup_full(a, e->stream));
+    e->exported_dec = e->exported_leaves = 0;
+    e->dec_base = e->sb_base = 0;
+    e->done_host = false;
+    e->dec_dev = 0;
+    e->initialized = true;
+    e->team_pending = e->team_E > 0 && e->cfg.team_nodes > 0;
+    return PBB_OK;
 }
 
 extern "C" int pbb_init_intervals(pbb_engine *e, int nb, const int *ids, const int *pos, const int *end)
@@ -509,36 +537,76 @@ extern "C" int pbb_init_intervals(pbb_engine *e, int nb, const int *ids, const i
     return setup_pool(e, idv, pos, end);
 }
 
+// One launch: empty the idle ring and clear the stop word (explorers idle at
+// the end of the previous launch register again when they run, so the ring
+// never names a wavefront that is not resident), then the explorers, either
+// the device-wide grid or the one-workgroup team; the control block and the
+// incumbent word come back into pinned memory behind them, and the host
+// waits once.  Returns the launch's device time in *ms.
+static int launch_once(pbb_engine *e, bool team, uint64_t node_budget, double time_budget_s, float *ms)
+{
+    *e->host_flag = 0;
+    KArgs a = make_args(e);
+    if (team) a.E = e->team_E;
+    a.node_budget = node_budget ? e->dec_dev + node_budget : 0;
+    a.deadline = time_budget_s > 0 ? (unsigned long long)(time_budget_s * e->wall_khz * 1000.0) : 0;
+    HIPCHK(launch_prelaunch(a, e->stream));
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    if (team) HIPCHK(launch_team(a, e->stream));
+    else HIPCHK(launch_explore(a, e->blocks, e->stream));
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipMemcpyAsync(e->ctl_h, e->ctl, sizeof(Ctl), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(e->best_h, e->best, 8, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipEventElapsedTime(ms, e->ev0, e->ev1));
+    const Ctl &c = *e->ctl_h;
+    e->done_host = c.done != 0;
+    e->dec_dev = c.dec;
+    if (c.err) return fail(PBB_E_HIP, "device error %d in the explorer kernel", c.err);
+    return take_incumbent(e, *e->best_h);
+}
+
+// A search started by pbb_init_full runs first on the team (one workgroup)
+// for up to cfg.team_nodes decomposed nodes; what is left then continues
+// on the device-wide grid from the team's saved intervals.
 extern "C" int pbb_run(pbb_engine *e, uint64_t node_budget, double time_budget_s, int *all_done)
 {
     if (!e) return fail(PBB_E_ARG, "null engine");
     if (!e->initialized) return fail(PBB_E_STATE, "pbb_run before pbb_init_full/pbb_init_intervals");
-    Ctl c;
-    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
-    if (c.done) {
+    if (all_done) *all_done = 0;
+    if (e->done_host) {
         if (all_done) *all_done = 1;
         return PBB_OK;
     }
-    c.stop = 0;
-    HIPCHK(hipMemcpy(&e->ctl->stop, &c.stop, sizeof(int), hipMemcpyHostToDevice));
-    int rc0 = reset_ring(e);
-    if (rc0) return rc0;
-    *e->host_flag = 0;
-    KArgs a = make_args(e);
-    a.node_budget = node_budget ? c.dec + node_budget : 0;
-    a.deadline = time_budget_s > 0 ? (unsigned long long)(time_budget_s * e->wall_khz * 1000.0) : 0;
-    HIPCHK(hipEventRecord(e->ev0, e->stream));
-    HIPCHK(launch_explore(a, e->blocks, e->stream));
-    HIPCHK(hipEventRecord(e->ev1, e->stream));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    e->last_ms = 0.0;
     float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
-    e->last_ms = ms;
-    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
-    if (c.err) return fail(PBB_E_HIP, "device error %d in the explorer kernel", c.err);
-    int rc = sync_incumbent(e);
+    if (e->team_pending) {
+        e->team_pending = false;
+        uint64_t tb = (uint64_t)e->cfg.team_nodes;
+        if (node_budget && node_budget < tb) tb = node_budget;
+        const uint64_t before = e->dec_dev;
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = launch_once(e, true, tb, time_budget_s, &ms);
+        if (rc) return rc;
+        e->last_ms += ms;
+        if (e->done_host) {
+            if (all_done) *all_done = 1;
+            return PBB_OK;
+        }
+        const uint64_t used = e->dec_dev - before;
+        if (node_budget) {
+            if (used >= node_budget) return PBB_OK;
+            node_budget -= used;
+        }
+        if (time_budget_s > 0) {
+            time_budget_s -= std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (time_budget_s <= 0) return PBB_OK;
+        }
+    }
+    int rc = launch_once(e, false, node_budget, time_budget_s, &ms);
     if (rc) return rc;
-    if (all_done) *all_done = c.done ? 1 : 0;
+    e->last_ms += ms;
+    if (all_done) *all_done = e->done_host ? 1 : 0;
     return PBB_OK;
 }
 
@@ -548,9 +616,9 @@ static int export_intervals(pbb_engine *e, int *pos, int *end, int *ids, int *nb
     const int n = e->n, E = e->E;
     std::vector<int> st(E);
     std::vector<int8_t> hp((size_t)E * 64), he((size_t)E * 64);
-    HIPCHK(hipMemcpy(st.data(), e->state, (size_t)E * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(hp.data(), e->pos, hp.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(he.data(), e->end, he.size(), hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, st.data(), e->state, (size_t)E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, hp.data(), e->pos, hp.size(), hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, he.data(), e->end, he.size(), hipMemcpyDeviceToHost));
     memset(pos, 0, sizeof(int) * (size_t)max * n);
     memset(end, 0, sizeof(int) * (size_t)max * n);
     int k = 0;
@@ -575,7 +643,7 @@ extern "C" int pbb_get_intervals(pbb_engine *e, int *pos, int *end, int *ids, in
     int rc = export_intervals(e, pos, end, ids, nb, max);
     if (rc) return rc;
     Ctl c;
-    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, &c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
     if (decomposed) *decomposed = c.dec - e->exported_dec;
     if (leaves) *leaves = c.leaves - e->exported_leaves;
     e->exported_dec = c.dec;
@@ -604,7 +672,7 @@ extern "C" int pbb_set_incumbent(pbb_engine *e, int cost, const int *perm)
     e->inc_cost = cost;
     if (perm) e->inc_perm.assign(perm, perm + e->n);
     unsigned long long packed = ((unsigned long long)(unsigned)cost << 32) | 0xffffffffull;
-    HIPCHK(hipMemcpy(e->best, &packed, 8, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->best, &packed, 8, hipMemcpyHostToDevice));
     return PBB_OK;
 }
 
@@ -612,7 +680,7 @@ extern "C" int pbb_get_stats(pbb_engine *e, pbb_stats *st)
 {
     if (!e || !st) return fail(PBB_E_ARG, "null argument");
     Ctl c;
-    HIPCHK(hipMemcpy(&c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, &c, e->ctl, sizeof c, hipMemcpyDeviceToHost));
     memset(st, 0, sizeof *st);
     st->decomposed = e->dec_base + c.dec;
     st->leaves = c.leaves;
@@ -620,7 +688,7 @@ extern "C" int pbb_get_stats(pbb_engine *e, pbb_stats *st)
     st->johnson_bounds = c.jb;
     st->explorers = e->E;
     std::vector<int> s(e->E);
-    HIPCHK(hipMemcpy(s.data(), e->state, (size_t)e->E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, s.data(), e->state, (size_t)e->E * 4, hipMemcpyDeviceToHost));
     int act = 0;
     for (int v : s) act += v != ST_IDLE;
     st->active = act;
@@ -640,11 +708,11 @@ extern "C" int pbb_get_deep_subproblems(pbb_engine *e, int *perms, int max, int 
     const int n = e->n, m = e->m, E = e->E;
     std::vector<int> st(E), line(E);
     std::vector<int8_t> pos((size_t)E * 64), dir((size_t)E * 64), mat((size_t)E * n * 64);
-    HIPCHK(hipMemcpy(st.data(), e->state, (size_t)E * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(line.data(), e->line, (size_t)E * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(pos.data(), e->pos, pos.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(dir.data(), e->dir, dir.size(), hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(mat.data(), e->mat, mat.size(), hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, st.data(), e->state, (size_t)E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, line.data(), e->line, (size_t)E * 4, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, pos.data(), e->pos, pos.size(), hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, dir.data(), e->dir, dir.size(), hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(e, mat.data(), e->mat, mat.size(), hipMemcpyDeviceToHost));
     std::vector<std::pair<int, int>> order;  // (cost, explorer)
     std::vector<int> sched((size_t)E * n);
     for (int i = 0; i < E; i++) {
@@ -701,13 +769,13 @@ static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub, const int 
 {
     build_tables(ptm, e->n, e->m, e->T);
     const HostTables &T = e->T;
-    HIPCHK(hipMemcpy(e->D.ptm, T.ptm.data(), T.ptm.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->D.ptmT, T.ptmT.data(), T.ptmT.size() * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->D.min_arr, T.min_arr.data(), T.m * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(e->D.min_dep, T.min_dep.data(), T.m * 4, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->D.ptm, T.ptm.data(), T.ptm.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->D.ptmT, T.ptmT.data(), T.ptmT.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->D.min_arr, T.min_arr.data(), T.m * 4, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->D.min_dep, T.min_dep.data(), T.m * 4, hipMemcpyHostToDevice));
     if (T.npairs) {
-        HIPCHK(hipMemcpy(e->D.pair_ab, T.pair_ab.data(), T.npairs * 4, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(e->D.jtab, T.jtab.data(), T.jtab.size() * sizeof(int4), hipMemcpyHostToDevice));
+        HIPCHK(copy_sync(e, e->D.pair_ab, T.pair_ab.data(), T.npairs * 4, hipMemcpyHostToDevice));
+        HIPCHK(copy_sync(e, e->D.jtab, T.jtab.data(), T.jtab.size() * sizeof(int4), hipMemcpyHostToDevice));
     }
     e->inc_cost = ub;
     e->initial_ub = ub;
@@ -719,7 +787,7 @@ static int engine_retarget(pbb_engine *e, const int32_t *ptm, int ub, const int 
     e->initialized = false;
     *e->host_flag = 0;
     unsigned long long packed = ((unsigned long long)(unsigned)ub << 32) | 0xffffffffull;
-    HIPCHK(hipMemcpy(e->best, &packed, 8, hipMemcpyHostToDevice));
+    HIPCHK(copy_sync(e, e->best, &packed, 8, hipMemcpyHostToDevice));
     return PBB_OK;
 }
 

@@ -56,6 +56,8 @@ extern "C" int pbb_config_default(pbb_config *c)
     c->init_solution = 0;
     c->checkpoint_s = 1;
     c->balance_s = 1;
+    c->ramp_nodes = 0;
+    c->team_nodes = 8192;
     return PBB_OK;
 }
 
@@ -74,6 +76,8 @@ int pbb::validate(const pbb_config *c)
                     c->sorted_dfs);
     if (c->nb_ivms < 1) return fail(PBB_E_ARG, "nbIVMs must be >= 1");
     if (c->steal_margin < 0) return fail(PBB_E_ARG, "steal margin %d is negative", c->steal_margin);
+    if (c->ramp_nodes < 0) return fail(PBB_E_ARG, "ramp %d is negative", c->ramp_nodes);
+    if (c->team_nodes < 0) return fail(PBB_E_ARG, "team node budget %d is negative", c->team_nodes);
     // earlyStopJohnson (bound_fsp_strong.cpp:114-123, :150-154): the Johnson
     // bound stops at the first pair whose value exceeds the incumbent and
     // reorders its pairs by past success, state of each bound object.  With
@@ -174,9 +178,12 @@ extern "C" int pbb_config_load_ini(const char *path, pbb_config *c, char *proble
     c->branching = (int)ini_int(kv, "bb.adaptivebranchingmode", 3);
     c->checkpoint_s = (int)ini_int(kv, "time.checkpoint", 1);
     c->balance_s = (int)ini_int(kv, "time.balance", 1);
-    // engine-specific keys (not in the reference): refinement bound, split margin
+    // engine-specific keys (not in the reference): refinement bound, split
+    // margin, donation ramp, team phase
     c->refine = (int)ini_int(kv, "bb.refinementbound", 0);
     c->steal_margin = (int)ini_int(kv, "gpu.stealmargin", 6);
+    c->ramp_nodes = (int)ini_int(kv, "gpu.rampnodes", 0);
+    c->team_nodes = (int)ini_int(kv, "gpu.teamnodes", 8192);
     return validate(c);
 }
 

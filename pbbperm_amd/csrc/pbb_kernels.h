@@ -74,8 +74,7 @@ struct Ctl {
 // incumbent and bounded root row (one per engine, one per pool slot).
 struct InstRef {
     Ctl *ctl;
-    unsigned *ring_seq;
-    int *ring_id;
+    unsigned long long *ring;   // idle-ring cells: ticket << 32 | explorer + 1
     int ring_cap;
     int ub0;                    // initial UB (branching thresholds when not singleNode)
     unsigned long long *best;   // cost << 32 | explorer
@@ -98,10 +97,10 @@ struct KArgs {
     int8_t *dir;        // [E][64]
     int *line;          // [E]
     int *state;         // [E]
-    // idle ring (bounded MPMC cells)
-    int *ring_id;       // [ring_cap]
-    unsigned *ring_seq; // [ring_cap]
-    int ring_cap;       // power of two >= E
+    // idle ring (bounded MPMC cells; each one word, ticket << 32 | explorer + 1,
+    // so a cell moves with single relaxed atomics and no fence)
+    unsigned long long *ring;   // [ring_cap]
+    int ring_cap;               // power of two >= E
     Ctl *ctl;
     unsigned long long *best;   // cost << 32 | explorer
     int *best_perm;             // [E][64]
@@ -116,6 +115,12 @@ struct KArgs {
     // configuration
     int bound_mode, branching, refine, pairs_mode, find_all, single_node, initial_ub;
     int steal_margin;
+    unsigned ramp;      // nodes an explorer decomposes in a fresh interval before it donates
+};
+
+// The root order of k_bound_root, passed by value (no upload before the launch).
+struct RootPerm {
+    int p[64];
 };
 
 // LDS footprint.  Workgroup tables: the machine-major processing times
@@ -162,6 +167,15 @@ inline size_t explore_lds_bytes(const DevInst &I, bool lb2)
 {
     return lds_table_bytes(I.n, I.m, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) +
            PBB_EXPLORE_WAVES * lds_wave_bytes(I.n, I.m, lb2);
+}
+// The team launch (one workgroup) holds at most 16 explorers with LB1 only
+// (4 per SIMD, the register budget of the LB1 explorer) and 8 with a
+// refinement bound (whose explorer needs more than 128 registers).
+__device__ __host__ constexpr int team_waves_max(int ref) { return ref == 0 ? 16 : 8; }
+inline size_t team_lds_bytes(const DevInst &I, bool lb2, int waves)
+{
+    return lds_table_bytes(I.n, I.m, I.npairs, lb2 && jtab_in_lds(I.n, I.npairs)) +
+           (size_t)waves * lds_wave_bytes(I.n, I.m, lb2);
 }
 // LDS size handed to the occupancy query of the persistent kernels.  Their
 // explorers wait on one another (the idle ring, work_count), so every
@@ -227,7 +241,7 @@ struct PoolSlot {                 // device header of one instance slot
 
 // byte offsets inside one slot's arena (all 256-byte aligned)
 struct PoolLayout {
-    size_t hdr, ptm, ptmT, minA, minD, pair, key, jtab, ring_seq, ring_id, stride;
+    size_t hdr, ptm, ptmT, minA, minD, pair, key, jtab, ring, stride;
 };
 __device__ __host__ inline size_t pool_al(size_t x) { return (x + 255) & ~(size_t)255; }
 __device__ __host__ inline PoolLayout pool_layout(int nmax, int M, int pmax, int ring_cap)
@@ -243,8 +257,7 @@ __device__ __host__ inline PoolLayout pool_layout(int nmax, int M, int pmax, int
     L.pair = o;     o += pool_al((size_t)(pmax > 0 ? pmax : 1) * 4);
     L.key = o;      o += pool_al((size_t)(pmax > 0 ? pmax : 1) * nmax * 4);
     L.jtab = o;     o += pool_al((size_t)(pmax > 0 ? pmax : 1) * nmax * 16);
-    L.ring_seq = o; o += pool_al((size_t)ring_cap * 4);
-    L.ring_id = o;  o += pool_al((size_t)ring_cap * 4);
+    L.ring = o;     o += pool_al((size_t)ring_cap * 8);
     L.stride = o;
     return L;
 }
@@ -277,8 +290,9 @@ __device__ __host__ inline size_t pool_lds_bytes(int nmax, int M, int pmax, bool
 
 // per-M kernels (pbb_kernels.hip, one translation unit per M)
 template <int M> hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s);
+template <int M> hipError_t launch_team_m(const KArgs &a, hipStream_t s);
 template <int M> hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu);
-template <int M> hipError_t launch_root_m(const KArgs &a, const int *root_perm, hipStream_t s);
+template <int M> hipError_t launch_root_m(const KArgs &a, const RootPerm &root_perm, hipStream_t s);
 template <int M>
 hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perms, const int *l1,
                            const int *l2, int arg, int *o1, int *o2, hipStream_t s);
@@ -293,8 +307,16 @@ hipError_t launch_neh_m(const int *ptms, const long long *off, const int *ns, co
 
 // runtime-M dispatch (pbb_dispatch.cpp)
 hipError_t launch_explore(const KArgs &a, int blocks, hipStream_t s);
+// one workgroup of a.E (<= PBB_TEAM_WAVES) explorers
+hipError_t launch_team(const KArgs &a, hipStream_t s);
 hipError_t explore_occupancy(const KArgs &a, int *blocks_per_cu);
-hipError_t launch_root(const KArgs &a, const int *root_perm, hipStream_t s);
+hipError_t launch_root(const KArgs &a, const RootPerm &root_perm, hipStream_t s);
+// engine set-up kernels (pbb_setup.hip, one translation unit): the whole
+// search space to explorer 0 and every other explorer idle, with the
+// control block cleared; the idle ring emptied and the stop word cleared
+// before a launch
+hipError_t launch_setup_full(const KArgs &a, hipStream_t s);
+hipError_t launch_prelaunch(const KArgs &a, hipStream_t s);
 // which: 0 lb1 children, 1 lb1 node, 2 lb2 node, 3 makespan
 hipError_t launch_bounds(int which, const KArgs &a, int count, const int *perms, const int *l1,
                          const int *l2, int arg, int *o1, int *o2, hipStream_t s);

@@ -35,22 +35,20 @@ namespace pbb {
 #define PBB_EXPLORE_MIN_WAVES 4
 #endif
 
-template <int M, int REF>
-__global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_WAVES : 1) void k_explore(KArgs a)
+// The explorer loop of one wavefront (explorer e); SC is the memory scope of
+// the explorers it shares work with.
+template <int M, int REF, int SC>
+__device__ __forceinline__ void explore_loop(const KArgs &a, const Lds &lds, int e)
 {
-    extern __shared__ __align__(16) char smem[];
-    Lds lds = lds_setup(a.I, smem, REF == 2, REF == 2);
-    const int e = blockIdx.x * PBB_EXPLORE_WAVES + (threadIdx.x >> 6);
-    if (e >= a.E) return;
-
-    Explorer<M, REF> X(a, lds, e, engine_inst(a), a.ctl, a.I.n, a.I.mp);
-    X.local_best = uni((int)(unsigned)(ld_rlx(a.best) >> 32));
-    int st = uni(ld_acq(a.state + e));
+    Explorer<M, REF, SC> X(a, lds, e, engine_inst(a), a.ctl, a.I.n, a.I.mp);
+    X.local_best = uni((int)(unsigned)(ld_rlx<SC>(a.best) >> 32));
+    // what the host or the previous launch left is visible at launch start
+    int st = uni(ld_rlx<SC>(a.state + e));
     if (st == ST_UNFOLD) {
         if (X.start_unfold()) {
             st = ST_ACTIVE;
         } else {
-            if (X.go_idle() && X.lane == 0) st_rel(&a.ctl->done, 1);
+            if (X.go_idle() && X.lane == 0) st_rel<SC>(&a.ctl->done, 1);
             st = ST_IDLE;
         }
     } else if (st == ST_ACTIVE) {
@@ -76,39 +74,41 @@ __global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_
             while (true) {
                 // relaxed poll (an acquire would invalidate the L2 at every
                 // pass); the fence pairs with the donor's release of the word
-                int s = uni(ld_rlx(a.state + e));
+                int s = uni(ld_rlx<SC>(a.state + e));
                 if (s == ST_ACTIVE) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                    fence_acq<SC>();
                     st = ST_ACTIVE;
                     X.load_state();
                     // a donated interval is explored under the current incumbent,
                     // not the one this explorer saw before it went idle
-                    X.take_best(ld_rlx(a.best));
+                    X.take_best(ld_rlx<SC>(a.best));
                     break;
                 }
-                if (uni(ld_rlx(&a.ctl->done)) || uni(ld_rlx(&a.ctl->stop)) ||
+                if (uni(ld_rlx<SC>(&a.ctl->done)) || uni(ld_rlx<SC>(&a.ctl->stop)) ||
                     (X.t_end && wall_clock64() >= X.t_end)) {
                     X.flush_counters();
                     return;
                 }
                 if (a.host_flag && (e & 15) == 0 && nap == 16 && X.lane == 0 &&
                     __hip_atomic_load(a.host_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM))
-                    st_rlx(&a.ctl->stop, 1);
-                X.idle_pause(nap);
+                    st_rlx<SC>(&a.ctl->stop, 1);
+                if constexpr (SC == __HIP_MEMORY_SCOPE_WORKGROUP) __builtin_amdgcn_s_sleep(2);
+                else X.idle_pause(nap);
             }
         }
-        // idle-ring poll issued before the node's work so its latency is hidden
-        const bool poll = hungry || (iter & (PBB_POLL_EVERY - 1)) == 0u;
+        // idle-ring poll issued before the node's work so its latency is
+        // hidden; none while the ramp holds this explorer's work back
+        const bool poll = X.since >= a.ramp && (hungry || (iter & (PBB_POLL_EVERY - 1)) == 0u);
         unsigned rh = 0, rt = 0;
         if (poll) {
-            rh = ld_rlx(&a.ctl->idle_head);
-            rt = ld_rlx(&a.ctl->idle_tail);
+            rh = ld_rlx<SC>(&a.ctl->idle_head);
+            rt = ld_rlx<SC>(&a.ctl->idle_tail);
         }
         const bool refresh = (iter & (PBB_BEST_EVERY - 1)) == 0u;
         unsigned long long bw = ~0ull;
-        if (refresh) bw = ld_rlx(a.best);
+        if (refresh) bw = ld_rlx<SC>(a.best);
         if (!X.step()) {
-            if (X.go_idle() && X.lane == 0) st_rel(&a.ctl->done, 1);
+            if (X.go_idle() && X.lane == 0) st_rel<SC>(&a.ctl->done, 1);
             st = ST_IDLE;
             continue;
         }
@@ -122,14 +122,40 @@ __global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_
     }
 }
 
+template <int M, int REF>
+__global__ __launch_bounds__(64 * PBB_EXPLORE_WAVES, REF == 0 ? PBB_EXPLORE_MIN_WAVES : 1) void k_explore(KArgs a)
+{
+    extern __shared__ __align__(16) char smem[];
+    Lds lds = lds_setup(a.I, smem, REF == 2, REF == 2);
+    const int e = blockIdx.x * PBB_EXPLORE_WAVES + (threadIdx.x >> 6);
+    if (e >= a.E) return;
+    explore_loop<M, REF, __HIP_MEMORY_SCOPE_AGENT>(a, lds, e);
+}
+
+// The team launch: explorers 0..a.E-1 as the wavefronts of one workgroup
+// (up to team_waves_max(REF)), sharing work with workgroup-scope ordering only.
+// A fresh search starts here: a small tree (a few thousand nodes) is solved
+// by the team alone, without the cross-XCD writebacks and invalidates that
+// every hand-over costs in a device-wide launch; a larger one stops at the
+// node budget and the device-wide launch resumes from the saved intervals.
+template <int M, int REF>
+__global__ __launch_bounds__(64 * team_waves_max(REF)) void k_team(KArgs a)
+{
+    extern __shared__ __align__(16) char smem[];
+    Lds lds = lds_setup(a.I, smem, REF == 2, REF == 2);
+    const int e = threadIdx.x >> 6;
+    if (e >= a.E) return;
+    explore_loop<M, REF, __HIP_MEMORY_SCOPE_WORKGROUP>(a, lds, e);
+}
+
 template <int M>
-__global__ __launch_bounds__(64) void k_bound_root(KArgs a, const int *root_perm)
+__global__ __launch_bounds__(64) void k_bound_root(KArgs a, RootPerm root_perm)
 {
     extern __shared__ __align__(16) char smem[];
     Lds lds = lds_setup(a.I, smem, false, false);
     Explorer<M, 0> X(a, lds, 0, engine_inst(a), a.ctl, a.I.n, a.I.mp);
     X.local_best = a.initial_ub;
-    X.rowv = X.lane < X.n ? root_perm[X.lane] : 0;
+    X.rowv = X.lane < X.n ? root_perm.p[X.lane] : 0;
     X.reset_stacks();
     X.bound_row(true);
     int d0 = rdlane(X.dirv, 0);
@@ -297,6 +323,25 @@ hipError_t launch_explore_m(const KArgs &a, int blocks, hipStream_t s)
 }
 
 template <int M>
+hipError_t launch_team_m(const KArgs &a, hipStream_t s)
+{
+    const bool lb2 = a.bound_mode != 0 && a.refine == 1;
+    const size_t lds = team_lds_bytes(a.I, lb2, a.E);
+    const void *k = a.bound_mode == 0 ? (const void *)k_team<M, 0>
+                    : a.refine == 1   ? (const void *)k_team<M, 2>
+                                      : (const void *)k_team<M, 1>;
+    if (const hipError_t err = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds))
+        return err;
+    if (a.bound_mode == 0)
+        hipLaunchKernelGGL((k_team<M, 0>), dim3(1), dim3(64 * a.E), lds, s, a);
+    else if (a.refine == 1)
+        hipLaunchKernelGGL((k_team<M, 2>), dim3(1), dim3(64 * a.E), lds, s, a);
+    else
+        hipLaunchKernelGGL((k_team<M, 1>), dim3(1), dim3(64 * a.E), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int M>
 hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
 {
     const bool lb2 = a.bound_mode != 0 && a.refine == 1;
@@ -317,7 +362,7 @@ hipError_t occupancy_m(const KArgs &a, int *blocks_per_cu)
 }
 
 template <int M>
-hipError_t launch_root_m(const KArgs &a, const int *root_perm, hipStream_t s)
+hipError_t launch_root_m(const KArgs &a, const RootPerm &root_perm, hipStream_t s)
 {
     const size_t lds = root_lds_bytes(a.I);
     if (const hipError_t err =
@@ -343,8 +388,9 @@ hipError_t launch_bounds_m(int which, const KArgs &a, int count, const int *perm
 
 #ifdef PBB_M
 template hipError_t launch_explore_m<PBB_M>(const KArgs &, int, hipStream_t);
+template hipError_t launch_team_m<PBB_M>(const KArgs &, hipStream_t);
 template hipError_t occupancy_m<PBB_M>(const KArgs &, int *);
-template hipError_t launch_root_m<PBB_M>(const KArgs &, const int *, hipStream_t);
+template hipError_t launch_root_m<PBB_M>(const KArgs &, const RootPerm &, hipStream_t);
 template hipError_t launch_bounds_m<PBB_M>(int, const KArgs &, int, const int *, const int *, const int *,
                                           int, int *, int *, hipStream_t);
 #endif

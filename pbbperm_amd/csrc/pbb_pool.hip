@@ -83,8 +83,7 @@ __device__ __forceinline__ InstRef slot_ref(const PoolArgs &p, int s, PoolSlot *
     char *b = slot_base(p, s);
     InstRef r;
     r.ctl = &S->c;
-    r.ring_seq = (unsigned *)(b + p.lay.ring_seq);
-    r.ring_id = (int *)(b + p.lay.ring_id);
+    r.ring = (unsigned long long *)(b + p.lay.ring);
     r.ring_cap = p.ring_cap;
     r.ub0 = ub;
     r.best = &S->best;
@@ -194,8 +193,8 @@ static __device__ __forceinline__ bool pool_admit(const PoolArgs &p, int s, bool
     int *ci = (int *)&S->c;
     const int wc = (int)(offsetof(Ctl, work_count) / 4);
     for (int i = tid; i < (int)(sizeof(Ctl) / 4); i += nt) ci[i] = (i == wc) ? 1 : 0;
-    unsigned *rs = (unsigned *)(b + p.lay.ring_seq);
-    for (int i = tid; i < p.ring_cap; i += nt) rs[i] = (unsigned)i;
+    unsigned long long *rs = (unsigned long long *)(b + p.lay.ring);
+    for (int i = tid; i < p.ring_cap; i += nt) rs[i] = (unsigned long long)(unsigned)i << 32;
     if (tid == 0) {
         S->best = ((unsigned long long)(unsigned)ub << 32) | 0xffffffffull;
         S->t_admit = wall_clock64();

@@ -134,6 +134,39 @@ def test_explorer_count_invariance(oracle_mod):
         assert best == opt and _same_counts(st, ref)
 
 
+@pytest.mark.parametrize("mode", [(0, 3, 0, 0), (2, 1, 1, 0)])
+def test_team_phase_hands_over_to_the_device(oracle_mod, mode):
+    """A search from pbb_init_full starts on the one-workgroup team and, past
+    its node budget, continues on the whole device from the team's saved
+    intervals: with the incumbent frozen, counts equal the oracle's for every
+    team budget (none, one that stops the team after its first check, one
+    the tree outgrows, one it never reaches), with and without the donation
+    ramp, and for a node-budgeted pbb_run that stops inside the team."""
+    import pbbperm_amd as P
+    O = oracle_mod
+    bm, br, rf, jp = mode
+    ptm = O.synth_ptm(1, 20, 10)
+    opt = _opt(O, ptm)
+    ref = O.solve(ptm, opt, bound_mode=bm, branching=br, refine=rf, johnson_pairs=jp)
+    print(f"mode={mode} reference decomposed={ref.decomposed}", flush=True)
+    for team, ramp, nb in [(0, 0, 4096), (64, 0, 4096), (2048, 0, 4096), (1 << 30, 0, 4096), (2048, 48, 4096),
+                           (2048, 0, 3)]:
+        best, _, st = _gpu(P, ptm, opt, bm, br, rf, jp, nb=nb, team_nodes=team, ramp_nodes=ramp)
+        print(f"  team={team} ramp={ramp} nb={nb} dec={st.decomposed}", flush=True)
+        assert best == opt and _same_counts(st, ref)
+    # a node budget smaller than the team's: the first pbb_run returns from
+    # the team, the next ones continue on the device
+    eng = P.GpuBB(ptm, _cfg(P, bm, br, rf, jp, 4096, team_nodes=8192), ub=opt)
+    eng.init_full_interval()
+    runs = 1
+    while not eng.next(node_budget=500, time_budget=60.0):
+        runs += 1
+        assert runs < 100000
+    st = eng.stats()
+    eng.close()
+    assert runs > 1 and _same_counts(st, ref)
+
+
 def test_full_grid_at_the_lds_edge():
     """12x20 with LB2 and the Johnson table in LDS fills a CU's LDS to within
     2 KB at three workgroups.  The persistent grid must still be co-resident:
@@ -479,7 +512,11 @@ def test_host_coworker_shares_one_search_with_the_engine(oracle_mod):
         assert eng.next(time_budget=60.0)
         t.join()
         assert res["cpu"]
-        assert eng.stats().decomposed + cpu.stats().decomposed == want
+        # the engine's counters restart at pbb_init_intervals: the first
+        # segment (dec1) is added back, as in test_export_import_intervals
+        gdec, cdec = eng.stats().decomposed, cpu.stats().decomposed
+        print(f"  first segment {dec1}, engine {gdec}, host {cdec}, oracle total {want}", flush=True)
+        assert dec1 + gdec + cdec == want
         eng.close()
         cpu.close()
     ptm = O.synth_ptm(42, 16, 8)
