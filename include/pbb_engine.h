@@ -383,6 +383,23 @@ int pbb_neh_device(int count, const int32_t *ptms, const int *n, const int *m, i
 int pbb_neh_device_noalloc(int count, const int32_t *d_ptms, const int *d_n, const int *d_m, int max_m,
                            int *d_perms, int *d_costs, void *stream);
 
+/* Iterated greedy on the device (heuristic/src/IG.cpp IG::runIG with the
+ * k-insertion local search of heuristic/src/fspnhoods.cpp:46-100, run as the
+ * reference's initial heuristic runs it: independent chains from NEH, the
+ * best kept, common/src/pbab.cpp:124-188).  For each of count instances
+ * (times laid out as in pbb_neh_device, 3 <= n <= 63, 2 <= m <= 20) run
+ * `chains` chains of `iterations` rounds each, one wavefront per chain.
+ * Chain c of instance i draws from a SplitMix64 stream seeded with
+ * seed ^ (i << 32 | c) (the reference seeds from std::random_device), and a
+ * worse solution is accepted with probability exp(-delta / T), T = 0.2 x
+ * the mean processing time / 10, through integer thresholds computed here
+ * (deltas >= 256 are refused).  perms: count x 64 ints (instance i's best
+ * order at perms[64 i ..]); costs: count best makespans, never above NEH's;
+ * chain_costs / chain_perms (may be NULL): count x chains costs and
+ * count x chains x 64 orders of every chain. */
+int pbb_ig_device(int count, const int32_t *ptms, const int *n, const int *m, int chains, int iterations,
+                  uint64_t seed, int *perms, int *costs, int *chain_costs, int *chain_perms);
+
 /* Batch: count <= slots instances, instance i's m[i] x n[i] machine-major
  * times at ptms[off_i], off_i = sum of n[k]*m[k] for k < i.  Upload, reset,
  * the resident launch and the result download are one graph launch.

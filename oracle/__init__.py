@@ -70,6 +70,15 @@ def lib():
         L.orc_bb_share.argtypes = [C.c_void_p, C.c_void_p, C.c_int]
         L.orc_bb_depth_histo.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.orc_bounds_batch.argtypes = [C.c_void_p, C.c_int, IP, IP, IP, C.c_int, IP, IP, IP, IP, IP, IP]
+        L.orc_insert_makespans.argtypes = [C.c_void_p, IP, C.c_int, C.c_int, IP]
+        L.orc_remove_makespans.argtypes = [C.c_void_p, IP, C.c_int, IP]
+        L.orc_ig_visit_order.argtypes = [C.c_int, IP]
+        L.orc_ig_below.argtypes = [C.POINTER(C.c_uint64), C.c_uint32]
+        L.orc_ig_below.restype = C.c_uint32
+        L.orc_ig_chain_seed.argtypes = [C.c_uint64, C.c_int, C.c_int]
+        L.orc_ig_chain_seed.restype = C.c_uint64
+        L.orc_ig_thresholds.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.orc_ig.argtypes = [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_uint64), IP]
         _lib = L
     return _lib
 
@@ -165,6 +174,68 @@ class Instance:
         p = np.arange(self.n, dtype=np.intc)
         c = lib().orc_neh(self.h, _ip(p))
         return c, p
+
+    # ---- iterated greedy (heuristic/src/IG.cpp, fastinsertremove.cpp)
+    def insert_makespans(self, seq, job: int) -> np.ndarray:
+        s = np.ascontiguousarray(seq, dtype=np.intc)
+        out = np.zeros(len(s) + 1, np.intc)
+        lib().orc_insert_makespans(self.h, _ip(s), len(s), int(job), _ip(out))
+        return out
+
+    def remove_makespans(self, seq) -> np.ndarray:
+        s = np.ascontiguousarray(seq, dtype=np.intc)
+        out = np.zeros(len(s), np.intc)
+        lib().orc_remove_makespans(self.h, _ip(s), len(s), _ip(out))
+        return out
+
+    def ig_thresholds(self) -> np.ndarray:
+        thr = np.zeros(IG_THR, np.uint64)
+        lib().orc_ig_thresholds(self.h, thr.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return thr
+
+    def ig(self, state: int, iters: int) -> Tuple[int, np.ndarray]:
+        """IG.cpp runIG from NEH with the seeded stream `state` (see
+        orc_ig_chain_seed for the state of a device chain)."""
+        thr = self.ig_thresholds()
+        p = np.zeros(self.n, np.intc)
+        c = lib().orc_ig(self.h, C.c_uint64(state), int(iters), thr.ctypes.data_as(C.POINTER(C.c_uint64)), _ip(p))
+        return c, p
+
+
+IG_THR = 256
+
+
+def ig_visit_order(n: int) -> np.ndarray:
+    vo = np.zeros(n, np.intc)
+    lib().orc_ig_visit_order(n, _ip(vo))
+    return vo
+
+
+def ig_chain_seed(seed: int, inst: int, chain: int) -> int:
+    return int(lib().orc_ig_chain_seed(C.c_uint64(seed), int(inst), int(chain)))
+
+
+def ref_moves(ptm: np.ndarray, samples: int, seed: int):
+    """The reference's IG visit order and fastInsertRemove insert/remove
+    makespans on seeded partial sequences (oracle/_ref/ref_harness moves)."""
+    import tempfile
+    m, n = ptm.shape
+    with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+        f.write(f"{n} {m}\n" + "\n".join(" ".join(str(int(v)) for v in row) for row in ptm) + "\n")
+        path = f.name
+    try:
+        out = subprocess.run([REF_HARNESS, "moves", path, str(samples), str(seed)], capture_output=True, text=True,
+                             check=True).stdout.splitlines()
+    finally:
+        os.unlink(path)
+    vo = [int(x) for x in out[0].split()[1:]]
+    rows = []
+    for line in out[1:]:
+        head, seq, ins, rem = line.split("|")
+        ln, job = (int(x) for x in head.split())
+        rows.append({"job": job, "seq": [int(x) for x in seq.split()], "ins": [int(x) for x in ins.split()],
+                     "rem": [int(x) for x in rem.split()]})
+    return vo, rows
 
 
 # ---------------------------------------------------------------- B&B

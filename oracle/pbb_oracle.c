@@ -17,6 +17,7 @@
 #include "pbb_oracle.h"
 
 #include <limits.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -913,4 +914,292 @@ void orc_bounds_batch(const orc_inst *I, int count, const int *perms, const int 
         if (lb2) lb2[s] = orc_lb2_node(I, p, l1[s], l2[s], pairs_mode, NULL);
         if (mk) mk[s] = orc_makespan(I, p);
     }
+}
+
+/* ======================================================================
+ * Iterated greedy (heuristic/src/IG.cpp, fspnhoods.cpp, fastinsertremove.cpp)
+ *
+ * The reference draws from std::random_device-seeded engines and compares a
+ * float exp in its acceptance, so no run of it can be repeated.  This is the
+ * same sequence of steps driven by a seeded SplitMix64 stream, with the
+ * acceptance probabilities given as integer thresholds and the removal
+ * weights as integers; the engine's device IG follows these steps exactly.
+ * ====================================================================== */
+
+/* fastInsertRemove::insertMakespans (fastinsertremove.cpp:196-214, with
+ * computeHeads/computeTails/computeInser :75-134): out[i], i = 0..len, is the
+ * makespan of seq[0..len-1] with job inserted before position i.  The tail
+ * beyond the sequence is zero (the reference reads tail[k][len], which
+ * computeTails never writes).  pinned: the reference's insertMakespans on
+ * seeded sequences with that cell zeroed (oracle/_ref, tests/golden). */
+void orc_insert_makespans(const orc_inst *I, const int *seq, int len, int job, int *out)
+{
+    const int m = I->m;
+    for (int i = 0; i <= len; i++) {
+        int c[ORC_MAXM] = {0};
+        for (int t = 0; t < i; t++) {
+            int j = seq[t];
+            c[0] += P_(I, 0, j);
+            for (int k = 1; k < m; k++) c[k] = imax(c[k - 1], c[k]) + P_(I, k, j);
+        }
+        c[0] += P_(I, 0, job);
+        for (int k = 1; k < m; k++) c[k] = imax(c[k - 1], c[k]) + P_(I, k, job);
+        for (int t = i; t < len; t++) {
+            int j = seq[t];
+            c[0] += P_(I, 0, j);
+            for (int k = 1; k < m; k++) c[k] = imax(c[k - 1], c[k]) + P_(I, k, j);
+        }
+        out[i] = c[m - 1];
+    }
+}
+
+/* fastInsertRemove::removeMakespans (fastinsertremove.cpp:243-279): out[i],
+ * i = 0..len-1, is the makespan of seq without position i (len >= 2).
+ * pinned: the reference's removeMakespans on seeded sequences. */
+void orc_remove_makespans(const orc_inst *I, const int *seq, int len, int *out)
+{
+    int tmp[ORC_MAXN];
+    for (int i = 0; i < len; i++) {
+        int t = 0;
+        for (int q = 0; q < len; q++)
+            if (q != i) tmp[t++] = seq[q];
+        out[i] = partial_makespan(I, tmp, len - 1);
+    }
+}
+
+/* IG::IG visitOrder (IG.cpp:38-52): the middle position, then alternately
+ * the positions below and above it, outwards. */
+void orc_ig_visit_order(int n, int *vo)
+{
+    int start = n / 2, ind = 0;
+    for (int i = start; i < n; i++, ind += 2) vo[ind] = i;
+    ind = 1;
+    for (int i = start - 1; i >= 0; i--, ind += 2) vo[ind] = i;
+}
+
+/* uniform draw in [0, k), k >= 1, from the high 32 bits of the stream
+ * (multiply-shift with rejection of the biased low products) */
+uint32_t orc_ig_below(uint64_t *s, uint32_t k)
+{
+    for (;;) {
+        uint32_t x = (uint32_t)(orc_splitmix_next(s) >> 32);
+        uint64_t mm = (uint64_t)x * k;
+        uint32_t lo = (uint32_t)mm;
+        if (lo < k) {
+            uint32_t t = (uint32_t)(0u - k) % k;
+            if (lo < t) continue;
+        }
+        return (uint32_t)(mm >> 32);
+    }
+}
+
+uint64_t orc_ig_chain_seed(uint64_t seed, int inst, int chain)
+{
+    uint64_t s = seed ^ (((uint64_t)(uint32_t)inst << 32) | (uint32_t)chain);
+    return orc_splitmix_next(&s);
+}
+
+/* IG::acceptance (IG.cpp:319-338): a worse (or equal) cost delta >= 0 is
+ * accepted with probability exp(-delta / T), T = 0.2 * avgPT / 10, avgPT the
+ * mean processing time (IG.cpp:24-32); thr[d] = floor(2^32 exp(-d/T)) for
+ * d < ORC_IG_THR (thr[0] = 2^32: equal costs are always accepted), larger
+ * deltas are refused.  A draw x (the high 32 bits of the stream) accepts
+ * when x < thr[delta]. */
+void orc_ig_thresholds(const orc_inst *I, uint64_t *thr)
+{
+    double sum = 0.0;
+    for (int i = 0; i < I->n * I->m; i++) sum += I->ptm[i];
+    double T = 0.2 * (sum / ((double)I->n * I->m)) / 10.0;
+    for (int d = 0; d < ORC_IG_THR; d++) {
+        double p = T > 0.0 ? exp(-(double)d / T) : (d == 0 ? 1.0 : 0.0);
+        double v = floor(p * 4294967296.0);
+        thr[d] = v >= 4294967296.0 ? (1ull << 32) : (uint64_t)v;
+    }
+}
+
+static void seq_remove(int *seq, int *len, int pos)
+{
+    for (int p = pos; p < *len - 1; p++) seq[p] = seq[p + 1];
+    (*len)--;
+}
+
+static void seq_insert(int *seq, int *len, int pos, int job)
+{
+    for (int p = *len; p > pos; p--) seq[p] = seq[p - 1];
+    seq[pos] = job;
+    (*len)++;
+}
+
+/* fspnhood::kImove (fspnhoods.cpp:46-100) with fastInsertRemove::bestInsert
+ * (fastinsertremove.cpp:326-363) and bestRemove2 (:463-496): remove the job
+ * at pos, insert it at the best non-tabu position; while that is worse than
+ * the starting makespan cmax0 (and fewer than kmax tries), remove a
+ * non-tabu job drawn with weight (gain / total time)^2 and insert it best.
+ * The weight is w = q^2, q = floor(256 gain / total time) (gain <= total
+ * time, so q <= 256); with every weight zero the first non-tabu position
+ * is removed.  Returns the makespan of seq on exit. */
+static int ig_ki_move(const orc_inst *I, int *seq, int n, int pos, int kmax, int cmax0, const int *sumpt,
+                      uint64_t *rng)
+{
+    int len = n;
+    int tabu_jobs[ORC_MAXN], ntj = 0, tabu_pos;
+    int ms[ORC_MAXN + 1];
+    int rjob = seq[pos];
+    seq_remove(seq, &len, pos);
+    tabu_jobs[ntj++] = rjob;
+    tabu_pos = pos;
+    int cmax1 = 0;
+    for (int k = 0;;) {
+        orc_insert_makespans(I, seq, len, rjob, ms);
+        int best = INT_MAX, bp = 0;
+        for (int i = 0; i <= len; i++) {
+            if (i == tabu_pos) continue;
+            if (ms[i] < best) {
+                best = ms[i];
+                bp = i;
+            }
+        }
+        seq_insert(seq, &len, bp, rjob);
+        cmax1 = best;
+        if (cmax1 <= cmax0) break;
+        if (++k == kmax) break;
+        int old = partial_makespan(I, seq, len);
+        orc_remove_makespans(I, seq, len, ms);
+        uint32_t w[ORC_MAXN], W = 0;
+        int first_free = -1;
+        for (int i = 0; i < len; i++) {
+            int job = seq[i], tabu = 0;
+            for (int t = 0; t < ntj; t++) tabu |= tabu_jobs[t] == job;
+            w[i] = 0;
+            if (!tabu) {
+                if (first_free < 0) first_free = i;
+                uint64_t g = (uint64_t)(old - ms[i]);
+                if (sumpt[job] > 0) {
+                    uint64_t q = (g << 8) / (uint64_t)sumpt[job];
+                    w[i] = (uint32_t)(q * q);
+                }
+            }
+            W += w[i];
+        }
+        int pick = first_free < 0 ? 0 : first_free;
+        if (W > 0) {
+            uint32_t u = orc_ig_below(rng, W), acc = 0;
+            for (int i = 0; i < len; i++) {
+                acc += w[i];
+                if (u < acc) {
+                    pick = i;
+                    break;
+                }
+            }
+        }
+        cmax1 = ms[pick];
+        rjob = seq[pick];
+        seq_remove(seq, &len, pick);
+        tabu_jobs[ntj++] = rjob;
+        tabu_pos = pick;
+    }
+    return cmax1;
+}
+
+/* IG::localSearchKI (IG.cpp:566-610): first improvement over the positions
+ * in visit order, restarted after every improvement, until none improves. */
+static int ig_local_search(const orc_inst *I, int *seq, int n, int cost, int kmax, const int *vo,
+                           const int *sumpt, uint64_t *rng)
+{
+    int tmp[ORC_MAXN];
+    for (int it = 0; it < 10000; it++) {
+        int found = 0;
+        for (int j = 0; j < n; j++) {
+            memcpy(tmp, seq, sizeof(int) * n);
+            int c = ig_ki_move(I, tmp, n, vo[j], kmax, cost, sumpt, rng);
+            if (c < cost) {
+                cost = c;
+                memcpy(seq, tmp, sizeof(int) * n);
+                found = 1;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    return cost;
+}
+
+/* IG::runIG (IG.cpp:341-404): from the NEH order, `iters` rounds of
+ * destruction (d jobs at positions drawn from the first r of the visit
+ * order, r uniform in [d, n], IG.cpp:94-150), greedy re-insertion at the
+ * first best position (IG.cpp:295-317), local search, and acceptance; d
+ * restarts at 2 after an acceptance and grows to 5 (then wraps to 2) after
+ * refusals.  Returns the best makespan seen; perm receives its order.
+ * Restatement with its own seeded generator: parity pinned by the moves
+ * above against the reference, and by the properties of its result (no
+ * worse than NEH, the order realises the cost, optimal on small instances). */
+int orc_ig(const orc_inst *I, uint64_t state, int iters, const uint64_t *thr, int *perm)
+{
+    const int n = I->n, m = I->m;
+    int cur[ORC_MAXN], tmp[ORC_MAXN], best[ORC_MAXN], vo[ORC_MAXN], sumpt[ORC_MAXN];
+    int v[ORC_MAXN], removed[ORC_MAXN], ms[ORC_MAXN + 1];
+    for (int j = 0; j < n; j++) {
+        cur[j] = j;
+        sumpt[j] = 0;
+        for (int k = 0; k < m; k++) sumpt[j] += P_(I, k, j);
+    }
+    int curcost = orc_neh(I, cur);
+    int bestcost = curcost;
+    memcpy(best, cur, sizeof(int) * n);
+    orc_ig_visit_order(n, vo);
+    int kmax = 0;
+    while ((kmax + 1) * (kmax + 1) <= n) kmax++;
+    int d = 2;
+    for (int it = 0; it < iters && n > 2; it++) {
+        memcpy(tmp, cur, sizeof(int) * n);
+        int dd = d < n - 1 ? d : n - 1;
+        int r = dd + (int)orc_ig_below(&state, (uint32_t)(n - dd + 1));
+        for (int i = 0; i < r; i++) v[i] = vo[i];
+        for (int i = r - 1; i > 0; i--) {
+            int j = (int)orc_ig_below(&state, (uint32_t)(i + 1));
+            int t = v[i];
+            v[i] = v[j];
+            v[j] = t;
+        }
+        int gone[ORC_MAXN] = {0};
+        for (int i = 0; i < dd; i++) {
+            removed[i] = tmp[v[i]];
+            gone[v[i]] = 1;
+        }
+        int len = 0;
+        for (int i = 0; i < n; i++)
+            if (!gone[i]) tmp[len++] = tmp[i];
+        for (int i = 0; i < dd; i++) {
+            orc_insert_makespans(I, tmp, len, removed[i], ms);
+            int bc = INT_MAX, bp = 0;
+            for (int p = 0; p <= len; p++)
+                if (ms[p] < bc) {
+                    bc = ms[p];
+                    bp = p;
+                }
+            seq_insert(tmp, &len, bp, removed[i]);
+        }
+        int tc = ig_local_search(I, tmp, n, partial_makespan(I, tmp, n), kmax, vo, sumpt, &state);
+        int acc;
+        if (tc < curcost) {
+            acc = 1;
+        } else {
+            int delta = tc - curcost;
+            uint64_t x = orc_splitmix_next(&state) >> 32;
+            acc = delta < ORC_IG_THR && x < thr[delta];
+        }
+        if (acc) {
+            memcpy(cur, tmp, sizeof(int) * n);
+            curcost = tc;
+            d = 2;
+        } else if (++d > 5) {
+            d = 2;
+        }
+        if (tc < bestcost) {
+            bestcost = tc;
+            memcpy(best, tmp, sizeof(int) * n);
+        }
+    }
+    memcpy(perm, best, sizeof(int) * n);
+    return bestcost;
 }

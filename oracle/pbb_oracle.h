@@ -120,6 +120,34 @@ int orc_bb_share(orc_bb *victim, orc_bb *thief, int min_depth_margin);
 /* Per-depth histogram of decomposed nodes (depth = line after goDown). */
 void orc_bb_depth_histo(const orc_bb *B, uint64_t *histo);
 
+/* ---------------- iterated greedy ---------------- */
+
+/* fastInsertRemove::insertMakespans / removeMakespans
+ * (heuristic/src/fastinsertremove.cpp:196-279): makespans of seq[0..len-1]
+ * with job inserted before each position 0..len, and without each position.
+ * pinned: the reference's own functions (oracle/_ref), tests/golden/ig_moves.json */
+void orc_insert_makespans(const orc_inst *I, const int *seq, int len, int job, int *out);
+void orc_remove_makespans(const orc_inst *I, const int *seq, int len, int *out);
+/* IG::IG visitOrder, heuristic/src/IG.cpp:38-52.  pinned: the reference's
+ * order, printed by oracle/_ref */
+void orc_ig_visit_order(int n, int *vo);
+/* The IG's generator: uniform in [0, k) from the SplitMix64 stream, and the
+ * stream state of chain `chain` of instance `inst`. */
+uint32_t orc_ig_below(uint64_t *state, uint32_t k);
+uint64_t orc_ig_chain_seed(uint64_t seed, int inst, int chain);
+/* Acceptance thresholds of IG::acceptance (IG.cpp:319-338) for cost deltas
+ * 0..ORC_IG_THR-1 (see pbb_oracle.c). */
+#define ORC_IG_THR 256
+void orc_ig_thresholds(const orc_inst *I, uint64_t *thr);
+/* IG::runIG (heuristic/src/IG.cpp:341-404) from the NEH order, `iters`
+ * rounds, driven by the SplitMix64 stream `state` (the reference's
+ * generators are seeded from std::random_device).  Returns the best
+ * makespan; perm receives its order.  pinned: the moves above against the
+ * reference + properties (<= NEH, the order realises the cost, optimum on
+ * small instances by brute force); the seeded steps themselves have no
+ * reference run to match (parity unpinned beyond that). */
+int orc_ig(const orc_inst *I, uint64_t state, int iters, const uint64_t *thr, int *perm);
+
 /* Batched form of the bound functions above (NULL outputs are skipped). */
 void orc_bounds_batch(const orc_inst *I, int count, const int *perms, const int *l1, const int *l2,
                       int pairs_mode, int *cb, int *ce, int *cbf, int *lb1n, int *lb2, int *mk);

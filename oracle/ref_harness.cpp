@@ -1,6 +1,7 @@
 // ref_harness.cpp -- drives the reference's own compiled sources (bounds/,
 // multicore/src/ivm.cpp, multicore/src/ivm_bound.cpp, common/src/{solution,
-// subproblem,arguments}.cpp, heuristic/src/fastNEH.cpp), built unmodified from
+// subproblem,arguments}.cpp, heuristic/src/{fastNEH,IG,fspnhoods,
+// fastinsertremove}.cpp), built unmodified from
 // the jangmys/PBBPerm tree by oracle/Makefile into oracle/_ref/.
 //
 // TEST INFRASTRUCTURE ONLY: it writes golden vectors for tests/golden and is
@@ -22,6 +23,7 @@
 //                       [max_nodes] [time_limit_s] [pos... end...]
 //   ref_harness neh     <ptmfile>
 //   ref_harness taillard <id>
+//   ref_harness moves   <ptmfile> <nsamples> <seed>
 // ptmfile: "n m" followed by the m x n machine-major processing times.
 #include <chrono>
 #include <fcntl.h>
@@ -39,6 +41,7 @@
 #include <vector>
 
 #include "arguments.h"
+#include "IG.h"
 #include "fastNEH.h"
 #include "ivm.h"
 #include "ivm_bound.h"
@@ -177,6 +180,53 @@ static int cmd_neh(int argc, char **argv)
         neh.runNEH(perm.data(), cost);
         printf("%d", cost);
         for (int i = 0; i < n; i++) printf(" %d", perm[i]);
+        printf("\n");
+    }
+    return 0;
+}
+
+// The moves of the reference's iterated greedy (heuristic/src/IG.cpp,
+// fspnhoods.cpp, fastinsertremove.cpp): the IG object's visit order, then
+// for nsamples seeded partial sequences (len uniform in [2, n-1], a
+// Fisher-Yates order from the splitmix64 stream of `seed`, the job after the
+// sequence as the one to insert) fastInsertRemove::insertMakespans and
+// removeMakespans.  The tail table is zeroed before each call: insertMakespans
+// reads tail[k][len], a cell computeTails(perm, len) does not write, so the
+// insertion at the end adds zero as intended (the reference reads whatever
+// an earlier, longer sequence left there).
+// Output: "vo v0 v1 ...", then per sample
+// "len job | seq0 .. seq_len-1 | ins0 .. ins_len | rem0 .. rem_len-1".
+static int cmd_moves(int argc, char **argv)
+{
+    ptm_instance inst(read_file(argv[2]));
+    const int ns = atoi(argv[3]);
+    uint64_t st = strtoull(argv[4], nullptr, 10);
+    IG ig(&inst);
+    fastInsertRemove *mv = ig.nhood->m;
+    const int n = ig.nbJob;
+    printf("vo");
+    for (int i = 0; i < n; i++) printf(" %d", ig.visitOrder[i]);
+    printf("\n");
+    std::vector<int> perm(n), ms(n + 1);
+    std::vector<int> msv(n + 1);
+    for (int s = 0; s < ns; s++) {
+        for (int i = 0; i < n; i++) perm[i] = i;
+        for (int i = n - 1; i > 0; i--) {
+            int j = (int)sm_unif(&st, (uint64_t)(i + 1));
+            std::swap(perm[i], perm[j]);
+        }
+        int len = 2 + (int)sm_unif(&st, (uint64_t)(n - 2));
+        int job = perm[len];
+        for (int k = 0; k < mv->nbMachines; k++)
+            for (int j = 0; j < n; j++) mv->tail[k][j] = 0;
+        mv->insertMakespans(perm.data(), len, job, msv);
+        printf("%d %d |", len, job);
+        for (int i = 0; i < len; i++) printf(" %d", perm[i]);
+        printf(" |");
+        for (int i = 0; i <= len; i++) printf(" %d", msv[i]);
+        mv->removeMakespans(perm.data(), len, ms.data());
+        printf(" |");
+        for (int i = 0; i < len; i++) printf(" %d", ms[i]);
         printf("\n");
     }
     return 0;
@@ -384,6 +434,7 @@ int main(int argc, char **argv)
     if (cmd == "solve" && argc >= 9) return cmd_solve(argc, argv);
     if (cmd == "neh") return cmd_neh(argc, argv);
     if (cmd == "taillard") return cmd_taillard(argc, argv);
+    if (cmd == "moves" && argc >= 5) return cmd_moves(argc, argv);
     fprintf(stderr, "bad command\n");
     return 2;
 }

@@ -130,6 +130,7 @@ SIGNATURES = {
                                  C.POINTER(C.c_double)]),
     "pbb_neh_device": (C.c_int, [C.c_int, I32P, IP, IP, IP, IP]),
     "pbb_neh_device_noalloc": (C.c_int, [C.c_int, VP, VP, VP, C.c_int, VP, VP, VP]),
+    "pbb_ig_device": (C.c_int, [C.c_int, I32P, IP, IP, C.c_int, C.c_int, C.c_uint64, IP, IP, IP, IP]),
     "pbb_serve_start": (C.c_int, [VP, C.c_double]),
     "pbb_serve_submit": (C.c_int, [VP, I32P, C.c_int, C.c_int, C.c_int, C.c_int]),
     "pbb_serve_poll": (C.c_int, [VP, C.POINTER(Result), C.c_int, IP]),

@@ -41,7 +41,7 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     jobs = jobs or min(16, os.cpu_count() or 4)
     units = []
     for m in MACHINES:
-        for src in ("pbb_kernels", "pbb_pool"):
+        for src in ("pbb_kernels", "pbb_pool", "pbb_ig"):
             obj = os.path.join(BUILD, f"{src}_m{m}.o")
             units.append([HIPCC, *FLAGS, f"-DPBB_M={m}", "-c", os.path.join(CSRC, src + ".hip"), "-o", obj])
     for src in ("pbb_dispatch.cpp", "pbb_host.cpp", "pbb_io.cpp", "pbb_pool.cpp", "pbb_checkpoint.cpp", "pbb_cpu.cpp"):

@@ -86,4 +86,13 @@ hipError_t launch_neh(int m, const int *ptms, const long long *off, const int *n
 #undef C_
 }
 
+hipError_t launch_ig(int m, const int *ptms, const long long *off, const int *ns, const int *ms, int count,
+                     int chains, int iters, unsigned long long seed, const unsigned long long *thr, int *perms,
+                     int *costs, hipStream_t s)
+{
+#define C_(M) launch_ig_m<M>(ptms, off, ns, ms, count, chains, iters, seed, thr, perms, costs, s)
+    PBB_FOR_M(C_, m)
+#undef C_
+}
+
 }  // namespace pbb

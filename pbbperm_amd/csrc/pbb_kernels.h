@@ -305,6 +305,14 @@ template <int M>
 hipError_t launch_neh_m(const int *ptms, const long long *off, const int *ns, const int *ms, int count, int nmax,
                         int *perms, int *costs, hipStream_t s);
 
+// iterated greedy (pbb_ig.hip, one translation unit per M): acceptance
+// thresholds per instance for cost deltas 0..IG_THR-1
+#define IG_THR 256
+template <int M>
+hipError_t launch_ig_m(const int *ptms, const long long *off, const int *ns, const int *ms, int count, int chains,
+                       int iters, unsigned long long seed, const unsigned long long *thr, int *perms, int *costs,
+                       hipStream_t s);
+
 // runtime-M dispatch (pbb_dispatch.cpp)
 hipError_t launch_explore(const KArgs &a, int blocks, hipStream_t s);
 // one workgroup of a.E (<= PBB_TEAM_WAVES) explorers
@@ -325,5 +333,8 @@ hipError_t launch_pool(int M, const KArgs &a, const PoolArgs &p, int groups, int
 hipError_t pool_occupancy(int M, const KArgs &a, const PoolArgs &p, int *groups_per_cu);
 hipError_t launch_neh(int M, const int *ptms, const long long *off, const int *ns, const int *ms, int count,
                       int nmax, int *perms, int *costs, hipStream_t s);
+hipError_t launch_ig(int M, const int *ptms, const long long *off, const int *ns, const int *ms, int count,
+                     int chains, int iters, unsigned long long seed, const unsigned long long *thr, int *perms,
+                     int *costs, hipStream_t s);
 
 }  // namespace pbb

@@ -8,6 +8,7 @@
 #include <atomic>
 #include <chrono>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <deque>
 #include <iterator>
@@ -451,9 +452,12 @@ extern "C" int pbb_pool_batch_histograms(pbb_pool *p, int count, uint64_t *decom
     if (count == 0) return PBB_OK;
     // a batch admits request i into slot i; each slot's control block keeps
     // its histograms until the slot is admitted again
+    // one linear copy per slot (the strided 2D copy read zeros under a
+    // profiler's counter collection)
     std::vector<Ctl> c(count);
-    HIPCHK(hipMemcpy2D(c.data(), sizeof(Ctl), p->arena + p->lay.hdr, p->lay.stride, sizeof(Ctl), count,
-                       hipMemcpyDeviceToHost));
+    for (int i = 0; i < count; i++)
+        HIPCHK(hipMemcpy(&c[i], p->arena + (size_t)i * p->lay.stride + p->lay.hdr, sizeof(Ctl),
+                         hipMemcpyDeviceToHost));
     for (int i = 0; i < count; i++)
         for (int d = 0; d < 64; d++) {
             decomposed[(size_t)i * 64 + d] = c[i].histo[d];
@@ -828,4 +832,91 @@ extern "C" int pbb_neh_device(int count, const int32_t *ptms, const int *n, cons
     rc = run();
     cleanup();
     return rc;
+}
+
+// ---------------------------------------------------------- iterated greedy
+// IG::acceptance (heuristic/src/IG.cpp:319-338): thr[d] = floor(2^32 exp(-d/T)),
+// T = 0.2 * (mean processing time) / 10 (IG.cpp:24-32), thr[0] = 2^32.
+static void ig_thresholds(const int32_t *ptm, int n, int m, unsigned long long *thr)
+{
+    double sum = 0.0;
+    for (int i = 0; i < n * m; i++) sum += ptm[i];
+    const double T = 0.2 * (sum / ((double)n * m)) / 10.0;
+    for (int d = 0; d < IG_THR; d++) {
+        const double p = T > 0.0 ? std::exp(-(double)d / T) : (d == 0 ? 1.0 : 0.0);
+        const double v = std::floor(p * 4294967296.0);
+        thr[d] = v >= 4294967296.0 ? (1ull << 32) : (unsigned long long)v;
+    }
+}
+
+extern "C" int pbb_ig_device(int count, const int32_t *ptms, const int *n, const int *m, int chains, int iterations,
+                             uint64_t seed, int *perms, int *costs, int *chain_costs, int *chain_perms)
+{
+    if (count < 0 || chains < 1 || iterations < 0 || (count > 0 && (!ptms || !n || !m || !perms || !costs)))
+        return fail(PBB_E_ARG, "bad arguments");
+    if (count == 0) return PBB_OK;
+    if ((long long)count * chains > (1 << 24)) return fail(PBB_E_CAPACITY, "count x chains above 2^24");
+    int rc = check_device();
+    if (rc) return rc;
+    std::vector<long long> off(count);
+    std::vector<unsigned long long> thr((size_t)count * IG_THR);
+    long long tot = 0;
+    int M = 2;
+    for (int i = 0; i < count; i++) {
+        if ((rc = check_shape(ptms + tot, n[i], m[i]))) return rc;
+        if (n[i] < 3 || n[i] > PBB_POOL_ROW - 1) return fail(PBB_E_ARG, "instance %d: n = %d outside [3, 63]", i, n[i]);
+        off[i] = tot;
+        ig_thresholds(ptms + tot, n[i], m[i], thr.data() + (size_t)i * IG_THR);
+        tot += (long long)n[i] * m[i];
+        M = std::max(M, m[i]);
+    }
+    const size_t outs = (size_t)count * chains;
+    int32_t *dp = nullptr;
+    long long *doff = nullptr;
+    int *dn = nullptr, *dm = nullptr, *dperm = nullptr, *dcost = nullptr;
+    unsigned long long *dthr = nullptr;
+    auto cleanup = [&]() {
+        hipFree(dp);
+        hipFree(doff);
+        hipFree(dn);
+        hipFree(dm);
+        hipFree(dperm);
+        hipFree(dcost);
+        hipFree(dthr);
+    };
+    std::vector<int> cperm(outs * 64), ccost(outs);
+    auto run = [&]() -> int {
+        HIPCHK(hipMalloc(&dp, (size_t)tot * 4));
+        HIPCHK(hipMalloc(&doff, (size_t)count * 8));
+        HIPCHK(hipMalloc(&dn, (size_t)count * 4));
+        HIPCHK(hipMalloc(&dm, (size_t)count * 4));
+        HIPCHK(hipMalloc(&dperm, outs * 64 * 4));
+        HIPCHK(hipMalloc(&dcost, outs * 4));
+        HIPCHK(hipMalloc(&dthr, thr.size() * 8));
+        HIPCHK(hipMemcpy(dp, ptms, (size_t)tot * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(doff, off.data(), (size_t)count * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dn, n, (size_t)count * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm, m, (size_t)count * 4, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dthr, thr.data(), thr.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(launch_ig(M, dp, doff, dn, dm, count, chains, iterations, (unsigned long long)seed, dthr, dperm, dcost,
+                         nullptr));
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpy(cperm.data(), dperm, outs * 64 * 4, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ccost.data(), dcost, outs * 4, hipMemcpyDeviceToHost));
+        return PBB_OK;
+    };
+    rc = run();
+    cleanup();
+    if (rc) return rc;
+    for (int i = 0; i < count; i++) {
+        int bc = 0;
+        for (int c = 1; c < chains; c++)
+            if (ccost[(size_t)i * chains + c] < ccost[(size_t)i * chains + bc]) bc = c;
+        costs[i] = ccost[(size_t)i * chains + bc];
+        std::copy(cperm.begin() + ((size_t)i * chains + bc) * 64, cperm.begin() + ((size_t)i * chains + bc + 1) * 64,
+                  perms + (size_t)i * 64);
+    }
+    if (chain_costs) std::copy(ccost.begin(), ccost.end(), chain_costs);
+    if (chain_perms) std::copy(cperm.begin(), cperm.end(), chain_perms);
+    return PBB_OK;
 }

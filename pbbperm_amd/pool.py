@@ -192,6 +192,28 @@ def neh_device(ptms: Sequence[np.ndarray]):
     return [perms[64 * i:64 * i + int(ns[i])].copy() for i in range(len(ptms))], costs[:len(ptms)].copy()
 
 
+def ig_device(ptms: Sequence[np.ndarray], chains: int = 64, iterations: int = 100, seed: int = 0,
+              per_chain: bool = False):
+    """Iterated greedy (IG.cpp runIG) on the device, `chains` seeded chains
+    per instance -> (best orders list, best costs array); with per_chain also
+    (chain costs [count, chains], chain orders [count, chains, n] lists)."""
+    flat, ns, ms = _pack(ptms)
+    k = max(1, len(ptms))
+    perms = np.zeros(64 * k, dtype=np.intc)
+    costs = np.zeros(k, dtype=np.intc)
+    cc = np.zeros(k * chains, dtype=np.intc) if per_chain else None
+    cp = np.zeros(k * chains * 64, dtype=np.intc) if per_chain else None
+    check(load().pbb_ig_device(len(ptms), flat.ctypes.data_as(_lib.I32P), _ip(ns), _ip(ms), int(chains),
+                               int(iterations), C.c_uint64(seed), _ip(perms), _ip(costs),
+                               None if cc is None else _ip(cc), None if cp is None else _ip(cp)))
+    best = [perms[64 * i:64 * i + int(ns[i])].copy() for i in range(len(ptms))]
+    if not per_chain:
+        return best, costs[:len(ptms)].copy()
+    chain_perms = [[cp[(i * chains + c) * 64:(i * chains + c) * 64 + int(ns[i])].copy() for c in range(chains)]
+                   for i in range(len(ptms))]
+    return best, costs[:len(ptms)].copy(), cc[:len(ptms) * chains].reshape(len(ptms), chains), chain_perms
+
+
 def replay_routed(pools: Sequence["Pool"], ptms: Sequence[np.ndarray], ubs: Sequence[int],
                   arrivals: Sequence[float], time_budget: float = 0.0):
     """Serve one arrival stream through several pools resident together: each

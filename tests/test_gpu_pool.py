@@ -138,6 +138,31 @@ def test_batch_c4_512_instances_match_reference():
     print(f"batch of 512: {ms:.1f} ms")
 
 
+def test_batch_histograms_match_the_oracle(oracle_mod):
+    """pbb_pool_batch_histograms after a batch: with frozen incumbents, each
+    instance's decomposed-node and refinement histograms by depth equal the
+    oracle's (its depth histogram; refinements counted by the depth of the
+    bounded node), and they sum to the instance's counters -- the figures
+    the batched roofline is computed from."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    O = oracle_mod
+    ptms = _small_batch(O, 24, 700, nlo=9, nhi=13)
+    opts = [_opt(O, p) for p in ptms]
+    cfg = _cfg(P, 2, 3, 1, 0)
+    with PP.Pool(cfg, max_jobs=13, machines=5, slots=24, waves_per_group=2, explorers_per_instance=16) as pool:
+        res, _ = pool.solve(ptms, opts, time_budget=BUDGET)
+        hd, hr = pool.batch_histograms(len(ptms))
+    for p, u, r, h, g in zip(ptms, opts, res, hd, hr):
+        assert r.status == "solved"
+        assert int(h.sum()) == r.decomposed and int(g.sum()) == r.johnson_bounds
+        bb = O.BB(O.Instance(p), u, bound_mode=2, branching=3, refine=1)
+        bb.init_full()
+        bb.run()
+        ref = bb.depth_histo()
+        assert h[:len(ref)].tolist() == list(ref) and not h[len(ref):].any()
+
+
 def test_repeated_graph_launches_are_identical(oracle_mod):
     """The captured graph replayed several times, with other GPU work and
     host synchronisation between the replays, gives the same results."""

@@ -159,3 +159,91 @@ def test_oracle_findall_solves_match_reference(oracle_mod):
         assert got == want, (c, got)
         differ += oracle_mod.solve(ptm, c["ub"], **kw).decomposed != c["decomposed"]
     assert differ > len(cases) // 2
+
+
+def _ig_moves():
+    with open(os.path.join(GOLDEN, "ig_moves.json")) as f:
+        return json.load(f)["shapes"]
+
+
+def test_oracle_ig_moves_match_reference(oracle_mod):
+    """The oracle's iterated-greedy moves against the reference's own
+    fastInsertRemove (tests/golden/ig_moves.json, written by
+    make_ig_golden.py from oracle/_ref): the IG visit order, and the
+    insertion and removal makespans of every sampled partial sequence, for
+    n = 3..63 and m = 2..20."""
+    O = oracle_mod
+    for sh in _ig_moves():
+        ptm = O.synth_ptm(sh["seed"], sh["n"], sh["m"])
+        inst = O.Instance(ptm)
+        assert list(O.ig_visit_order(sh["n"])) == sh["visit_order"]
+        for s in sh["samples"]:
+            assert list(inst.insert_makespans(s["seq"], s["job"])) == s["ins"], (sh["n"], sh["m"], s)
+            assert list(inst.remove_makespans(s["seq"])) == s["rem"], (sh["n"], sh["m"], s)
+
+
+def test_oracle_ig_moves_end_positions_by_recomputation(oracle_mod):
+    """The fixture's end cases, recomputed from scratch by the textbook
+    recurrence: inserting the job after the whole sequence (the cell the
+    reference leaves unwritten, here zero), before it, and removing the
+    first and the last job; every insertion is no shorter and every removal
+    no longer than the sequence itself."""
+    O = oracle_mod
+    for sh in _ig_moves():
+        inst = O.Instance(O.synth_ptm(sh["seed"], sh["n"], sh["m"]))
+        for s in sh["samples"]:
+            seq, job = s["seq"], s["job"]
+            whole = _partial(inst, seq)
+            assert s["ins"][-1] == _partial(inst, seq + [job])
+            assert s["ins"][0] == _partial(inst, [job] + seq)
+            assert s["rem"][0] == _partial(inst, seq[1:]) and s["rem"][-1] == _partial(inst, seq[:-1])
+            assert min(s["ins"]) >= whole and max(s["rem"]) <= whole
+
+
+def _partial(inst, seq):
+    c = np.zeros(inst.m, np.int64)
+    for j in seq:
+        c[0] += inst.ptm[0, j]
+        for k in range(1, inst.m):
+            c[k] = max(c[k - 1], c[k]) + inst.ptm[k, j]
+    return int(c[-1])
+
+
+def test_oracle_ig_properties(oracle_mod):
+    """orc_ig (IG.cpp runIG with a seeded stream): never worse than NEH, its
+    order realises its cost, the same seed gives the same result, and on 40
+    tiny instances (n = 6..8) it reaches the brute-force optimum on at
+    least 38 (a broken move, acceptance or destruction loses many)."""
+    import itertools
+    O = oracle_mod
+    hits = 0
+    for seed in range(40):
+        n, m = 6 + seed % 3, 3 + seed % 4
+        ptm = O.synth_ptm(1000 + seed, n, m)
+        inst = O.Instance(ptm)
+        opt = min(inst.makespan(np.array(p, np.intc)) for p in itertools.permutations(range(n)))
+        st = O.ig_chain_seed(3, seed, 0)
+        c, p = inst.ig(st, 30)
+        assert (c, list(p)) == tuple((lambda r: (r[0], list(r[1])))(inst.ig(st, 30)))
+        neh, _ = inst.neh()
+        assert opt <= c <= neh and inst.makespan(p) == c and sorted(p) == list(range(n))
+        hits += c == opt
+    assert hits >= 38
+    for seed, n, m in [(0, 20, 5), (2, 50, 20), (11, 63, 20)]:
+        inst = O.Instance(O.synth_ptm(seed, n, m))
+        neh, _ = inst.neh()
+        c, p = inst.ig(O.ig_chain_seed(5, 0, 0), 3)
+        assert c <= neh and inst.makespan(p) == c
+
+
+def test_oracle_ig_thresholds(oracle_mod):
+    """Acceptance thresholds of IG::acceptance: exp(-d / T) scaled to 2^32,
+    T = 0.2 * mean time / 10, equal costs always accepted, decreasing."""
+    O = oracle_mod
+    ptm = O.synth_ptm(4, 20, 10)
+    thr = O.Instance(ptm).ig_thresholds()
+    T = 0.2 * ptm.mean() / 10.0
+    assert int(thr[0]) == 1 << 32
+    for d in (1, 2, 5):
+        assert abs(int(thr[d]) - np.floor(np.exp(-d / T) * 2.0 ** 32)) <= 1
+    assert all(int(a) >= int(b) for a, b in zip(thr[:-1], thr[1:]))
