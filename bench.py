@@ -63,13 +63,14 @@ PEAK_HBM_GBS = 8000.0
 # plain 256 MiB device copy): profiles/k_explore_50x20_lb1_pmc.md
 HBM_BYTES_PER_NODE = 0.80
 # HBM bytes of one k_pool<10,2> launch of the 512-instance batch, from the
-# same two counters and factors (None until measured)
-BATCH_HBM_BYTES = None
+# same two counters and factors: profiles/k_pool_batch512_lb2_pmc.md
+BATCH_HBM_BYTES = 3.13e9
 # The serving stream of the default run: requests, Poisson rate and the
-# per-request time budget after admission
+# per-request time budget after admission.  Every path's replay ends at the
+# arrival span + 2 budgets + 15 s; what is not served by then counts late.
 SERVE_REQUESTS = 600
 SERVE_RATE = 20.0
-SERVE_BUDGET_S = 10.0
+SERVE_BUDGET_S = 2.0
 
 
 def lb1_ops(depth: int, n: int, m: int) -> int:
@@ -359,6 +360,12 @@ def leg_batched(P, PP, args):
         ref_match = [r.best for r in res] == opts
     wall = min(walls)
     nodes = sum(r.decomposed for r in res)
+    # how the batch ends: when instances retire (device clock, ms after the
+    # earliest admission) -- the last few trees set the batch time
+    done = np.sort(np.array([r.done_ms for r in res]))
+    tail = {"retired_p50_ms": float(np.percentile(done, 50)), "retired_p90_ms": float(np.percentile(done, 90)),
+            "retired_p99_ms": float(np.percentile(done, 99)), "retired_last_ms": float(done[-1]),
+            "largest_trees": sorted((int(r.decomposed) for r in res), reverse=True)[:3]}
     # the roofline of k_pool<10,2>: the batch's algorithmic int32 ops over the
     # graph's device time (HIP events on the pool's stream around the graph:
     # the k_pool launch plus k_pool_init and the 2.6 MB request upload and
@@ -375,7 +382,7 @@ def leg_batched(P, PP, args):
         "workload": "512 instances, seeds 100-611, n~U[10,22], m~U{5,10}, LB1 + Johnson LB2 (all pairs), "
                     "MinMin branching, NEH incumbents, solved to optimality in one graph launch",
         "solves_per_s": len(ptms) / wall, "wall_ms": wall * 1e3, "graph_ms": min(kms),
-        "decomposed": nodes, "nodes_per_s": nodes / wall,
+        "decomposed": nodes, "nodes_per_s": nodes / wall, "tail": tail,
         "plain_solves_per_s": len(ptms) / pwall, "plain_wall_ms": pwall * 1e3,
         "plain_explorers_per_instance": best.explorers_per_instance,
         "speedup_vs_plain": pwall / wall,
@@ -408,7 +415,7 @@ def leg_serving(P, PP, args):
     rate = args.serve_rate
     arrivals = PP.poisson_arrivals(N, rate, seed=7)
     span = float(arrivals[-1]) if N else 0.0
-    tb = span + 4 * args.serve_budget + 60.0
+    tb = span + 2 * args.serve_budget + 15.0
     note(f"serving: {N} requests at {rate:.1f}/s ({span:.0f} s of arrivals), budget {args.serve_budget} s "
          f"per request, through the pool")
     with PP.Pool(cfg, params=best) as pool:
@@ -424,6 +431,19 @@ def leg_serving(P, PP, args):
 
     def agree(a, b):
         return all(x.best == y.best for x, y in zip(a, b) if x.status == "solved" and y.status == "solved")
+
+    # requests of the stream in the reference fixture (tests/golden/
+    # serve_wide.json: the first ones of each machine count and job band whose
+    # search a CPU finishes): every solved one against the reference's optimum
+    ref_check = None
+    fx = os.path.join(ROOT, "tests", "golden", "serve_wide.json")
+    if os.path.exists(fx):
+        with open(fx) as f:
+            want = {d["seed"]: d["opt"] for d in json.load(f)["instances"]}
+        pairs = [(r.best, want[10000 + i]) for path in (res, rres, plain) for i, r in enumerate(path)
+                 if 10000 + i in want and r.status == "solved"]
+        ref_check = {"checked": len(pairs), "equal": sum(a == b for a, b in pairs),
+                     "fixture_requests_in_stream": sum(1 for i in range(N) if 10000 + i in want)}
 
     by_m = {}
     for p, r, l in zip(ptms, res, lat):
@@ -442,6 +462,7 @@ def leg_serving(P, PP, args):
                    **_latency(rlat, rwall, rres), "optima_equal_plain": agree(rres, plain)},
         "plain": {**_latency(plat, pwall, plain), "explorers_per_instance": best.explorers_per_instance},
         "optima_equal_plain": agree(res, plain),
+        "optima_vs_reference": ref_check,
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
                   "steal_margin": best.steal_margin, "source": src},
     }
