@@ -42,6 +42,11 @@ __device__ __forceinline__ int wrlane(int old, int v, int l)
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// lanes 0..k-1 as a mask, computed in scalar registers (0 <= k <= 64).  A
+// ballot of `lane < k && cond` compiles to a VGPR 0/1 select and a second
+// compare; lanes_below(k) & ballot(cond) keeps cond's compare mask as is.
+__device__ __forceinline__ uint64_t lanes_below(int k) { return k >= 64 ? ~0ull : ((1ull << k) - 1ull); }
+
 __device__ __forceinline__ int shfl(int v, int src) { return __shfl(v, src, PBB_WAVE); }
 
 // Wavefront reductions on DPP lanes moves (row_shr 1,2,4,8 inside each

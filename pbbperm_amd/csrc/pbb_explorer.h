@@ -410,9 +410,9 @@ struct Explorer {
     // ivm::beforeEnd (ivm.cpp:152-160): lexicographic pos <= end
     __device__ __forceinline__ bool before_end() const
     {
-        bool in = lane < n;
-        uint64_t lt = ballot(in && posv < endv);
-        uint64_t gt = ballot(in && posv > endv);
+        const uint64_t in = lanes_below(n);
+        uint64_t lt = in & ballot(posv < endv);
+        uint64_t gt = in & ballot(posv > endv);
         uint64_t d = lt | gt;
         if (d == 0) return true;
         int f = __ffsll((unsigned long long)d) - 1;
@@ -534,7 +534,8 @@ struct Explorer {
         }
         case 2: {
             int ub = a.single_node ? local_best : ir.ub0;
-            int elim = __popcll(ballot(valid && cb >= ub)) - __popcll(ballot(valid && ce >= ub));
+            const uint64_t vm = ballot(valid);
+            int elim = __popcll(vm & ballot(cb >= ub)) - __popcll(vm & ballot(ce >= ub));
             int s = wave_sum(valid ? ((cb < ub ? cb : 0) - (ce < ub ? ce : 0)) : 0);
             if (elim > 0) return 0;
             if (elim < 0) return 1;
@@ -543,8 +544,9 @@ struct Explorer {
         default: {
             int ub = a.single_node ? local_best : ir.ub0;
             int mn = wave_min(valid ? min(cb > 0 ? cb : PBB_INF, ce > 0 ? ce : PBB_INF) : PBB_INF);
-            int minc = __popcll(ballot(valid && cb == mn)) - __popcll(ballot(valid && ce == mn));
-            int elimc = __popcll(ballot(valid && cb >= ub)) - __popcll(ballot(valid && ce >= ub));
+            const uint64_t vm = ballot(valid);
+            int minc = __popcll(vm & ballot(cb == mn)) - __popcll(vm & ballot(ce == mn));
+            int elimc = __popcll(vm & ballot(cb >= ub)) - __popcll(vm & ballot(ce >= ub));
             if (minc > 0) return 1;
             if (minc < 0) return 0;
             return elimc > 0 ? 0 : 1;
@@ -866,7 +868,7 @@ struct Explorer {
             int c = p;
             if (p < n - L) {
                 // first live cell at or after p (ivm::goRight over pruned / visited cells)
-                const uint64_t live = ballot(lane < n - L && rowv >= 0);
+                const uint64_t live = lanes_below(n - L) & ballot(rowv >= 0);
                 const uint64_t ahead = live & (~0ull << p);
                 c = ahead ? __ffsll((unsigned long long)ahead) - 1 : n - L;
                 if (c != p) {
@@ -1009,7 +1011,7 @@ struct Explorer {
         if (h == t) return false;
         const int lim = min(L, n - 1 - a.steal_margin);  // last line a split may use
         if (lim < 0) return true;                         // margin >= n: never split
-        uint64_t ne = ballot(lane < n && posv != endv);
+        uint64_t ne = lanes_below(n) & ballot(posv != endv);
         uint64_t limmask = lim >= 64 ? ~0ull : ((1ull << lim) - 1ull);
         uint64_t cand = ne & limmask;
         const int first = cand ? __ffsll((unsigned long long)cand) - 1 : lim;
@@ -1021,7 +1023,7 @@ struct Explorer {
             el = (l == first) ? rdlane(endv, l) : n - 1 - l;
             if (pl < el) {
                 rowl = (l == L) ? rowv : (lane < n - l ? (int)lds.mat[lds_mat_row(l, n) + lane] : -1);
-                expl = ballot(lane > pl && lane <= el && rowl >= 0);
+                expl = (lanes_below(el + 1) & ~lanes_below(pl + 1)) & ballot(rowl >= 0);
                 if (expl) break;
             }
         }
