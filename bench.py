@@ -61,7 +61,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per decomposed node of k_explore on this workload, from two
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, factors from a
 # plain 256 MiB device copy): profiles/k_explore_50x20_lb1_pmc.md
-HBM_BYTES_PER_NODE = 0.80
+HBM_BYTES_PER_NODE = 0.81
 # HBM bytes of one k_pool<10,2> launch of the 512-instance batch, from the
 # same two counters and factors: profiles/k_pool_batch512_lb2_pmc.md
 BATCH_HBM_BYTES = 3.13e9
@@ -416,20 +416,27 @@ def leg_serving(P, PP, args):
     arrivals = PP.poisson_arrivals(N, rate, seed=7)
     span = float(arrivals[-1]) if N else 0.0
     tb = span + 2 * args.serve_budget + 15.0
+    paths = set(args.serve_paths.split(","))
     note(f"serving: {N} requests at {rate:.1f}/s ({span:.0f} s of arrivals), budget {args.serve_budget} s "
-         f"per request, through the pool")
-    with PP.Pool(cfg, params=best) as pool:
-        res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=tb)
-    note(f"serving: pool done in {wall:.1f} s; routed pools")
+         f"per request, paths {sorted(paths)}")
+    res = rres = plain = None
+    if "pool" in paths:
+        with PP.Pool(cfg, params=best) as pool:
+            res, lat, wall = pool.replay(ptms, ubs, arrivals, time_budget=tb)
+        note(f"serving: pool done in {wall:.1f} s")
     rprm = PP.serving_routed_params(best)
-    with PP.Pool(cfg, params=rprm[1]) as p20, PP.Pool(cfg, params=rprm[0]) as p10:
-        rres, rlat, rwall = PP.replay_routed([p20, p10], ptms, ubs, arrivals, time_budget=tb)
-    note(f"serving: routed done in {rwall:.1f} s; plain path")
-    plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=pcfg, time_budget=tb,
-                                         request_budget=args.serve_budget)
-    note(f"serving: plain done in {pwall:.1f} s")
+    if "routed" in paths:
+        with PP.Pool(cfg, params=rprm[1]) as p20, PP.Pool(cfg, params=rprm[0]) as p10:
+            rres, rlat, rwall = PP.replay_routed([p20, p10], ptms, ubs, arrivals, time_budget=tb)
+        note(f"serving: routed done in {rwall:.1f} s")
+    if "plain" in paths:
+        plain, plat, pwall = PP.plain_replay(ptms, ubs, arrivals, config=pcfg, time_budget=tb,
+                                             request_budget=args.serve_budget)
+        note(f"serving: plain done in {pwall:.1f} s")
 
     def agree(a, b):
+        if a is None or b is None:
+            return None
         return all(x.best == y.best for x, y in zip(a, b) if x.status == "solved" and y.status == "solved")
 
     # requests of the stream in the reference fixture (tests/golden/
@@ -440,45 +447,64 @@ def leg_serving(P, PP, args):
     if os.path.exists(fx):
         with open(fx) as f:
             want = {d["seed"]: d["opt"] for d in json.load(f)["instances"]}
-        pairs = [(r.best, want[10000 + i]) for path in (res, rres, plain) for i, r in enumerate(path)
-                 if 10000 + i in want and r.status == "solved"]
+        pairs = [(r.best, want[10000 + i]) for path in (res, rres, plain) if path is not None
+                 for i, r in enumerate(path) if 10000 + i in want and r.status == "solved"]
         ref_check = {"checked": len(pairs), "equal": sum(a == b for a, b in pairs),
                      "fixture_requests_in_stream": sum(1 for i in range(N) if 10000 + i in want)}
 
-    by_m = {}
-    for p, r, l in zip(ptms, res, lat):
-        d = by_m.setdefault(int(p.shape[0]), {"requests": 0, "late": 0})
-        d["requests"] += 1
-        d["late"] += int(r.status != "solved" or l < 0)
-    return {
+    out = {
         "workload": f"{N} Poisson requests at {rate:g}/s, seeds 10000-{10000 + N - 1}, n~U[12,30], "
                     "m~U{5,10,20}, LB1 + Johnson LB2 (all pairs), MinMin, NEH incumbents, per-request budget "
                     f"{args.serve_budget:g} s after admission (over-budget requests are stopped and counted late)",
-        "requests": N, "rate_per_s": rate, "request_budget_s": args.serve_budget,
-        **_latency(lat, wall, res),
-        "late_by_machines": by_m,
-        "decomposed": int(sum(r.decomposed for r in res)),
-        "routed": {"pools": [{"machines": q.machines, "max_jobs": q.max_jobs, "groups": q.groups} for q in rprm],
-                   **_latency(rlat, rwall, rres), "optima_equal_plain": agree(rres, plain)},
-        "plain": {**_latency(plat, pwall, plain), "explorers_per_instance": best.explorers_per_instance},
+        "requests": N, "rate_per_s": rate, "request_budget_s": args.serve_budget, "paths": sorted(paths),
+    }
+    if res is not None:
+        by_m = {}
+        for p, r, l in zip(ptms, res, lat):
+            d = by_m.setdefault(int(p.shape[0]), {"requests": 0, "late": 0})
+            d["requests"] += 1
+            d["late"] += int(r.status != "solved" or l < 0)
+        out.update(_latency(lat, wall, res))
+        out["late_by_machines"] = by_m
+        out["decomposed"] = int(sum(r.decomposed for r in res))
+    if rres is not None:
+        out["routed"] = {"pools": [{"machines": q.machines, "max_jobs": q.max_jobs, "groups": q.groups} for q in rprm],
+                         **_latency(rlat, rwall, rres), "optima_equal_plain": agree(rres, plain),
+                         "optima_equal_pool": agree(rres, res)}
+    if plain is not None:
+        out["plain"] = {**_latency(plat, pwall, plain), "explorers_per_instance": best.explorers_per_instance}
+    out.update({
         "optima_equal_plain": agree(res, plain),
         "optima_vs_reference": ref_check,
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
                   "steal_margin": best.steal_margin, "source": src},
-    }
+    })
+    return out
 
 
 def leg_treesizes(P, PP, args):
     """Tree sizes of the serving stream's distribution (its first
     --tree-requests requests): every request solved alone in turn through a
     pool with the serving tuning and a per-request budget of
-    --tree-budget s, NEH incumbent.  Reports decomposed nodes per request
+    --tree-budget s, NEH incumbent, or with --tree-ub ig the device iterated
+    greedy's (--ig-chains chains of --ig-iters iterations per request, all
+    requests in one launch, timed).  Reports decomposed nodes per request
     (a lower bound for the requests the budget stopped) as a histogram by
     machine count and decade."""
     import numpy as np
     K = args.tree_requests
     ptms = PP.serving_stream(K, 10000)
-    ubs = [P.neh(p)[0] for p in ptms]
+    nehs = [P.neh(p)[0] for p in ptms]
+    ubs, ig = nehs, None
+    if args.tree_ub == "ig":
+        PP.ig_device(ptms[:1], chains=args.ig_chains, iterations=1, seed=1)  # load the code object
+        t = time.perf_counter()
+        _, costs = PP.ig_device(ptms, chains=args.ig_chains, iterations=args.ig_iters, seed=1)
+        ig_s = time.perf_counter() - t
+        ubs = [int(c) for c in costs]
+        ig = {"chains": args.ig_chains, "iterations": args.ig_iters, "seconds_all_requests": round(ig_s, 4),
+              "below_neh": sum(u < h for u, h in zip(ubs, nehs))}
+        note(f"tree sizes: IG incumbents in {ig_s:.3f} s, {ig['below_neh']} of {K} below NEH")
     best, _ = _params(PP, "serving", args)
     best.request_budget_s = args.tree_budget
     best.slots = 1
@@ -488,19 +514,21 @@ def leg_treesizes(P, PP, args):
             t = time.perf_counter()
             r, _ = pool.solve([p], [u], time_budget=args.tree_budget + 30.0)
             rows.append((int(p.shape[0]), int(p.shape[1]), r[0].status, int(r[0].decomposed),
-                         time.perf_counter() - t))
+                         time.perf_counter() - t, int(u), int(r[0].best)))
             if (i + 1) % 25 == 0:
                 note(f"tree sizes: {i + 1} / {K}")
     hist = {}
-    for m, n, st, dec, sec in rows:
+    for m, n, st, dec, sec, _, _ in rows:
         dcd = int(np.floor(np.log10(max(dec, 1))))
         key = f"m{m}"
         h = hist.setdefault(key, {})
         k2 = f"1e{dcd}" + ("+" if st != "solved" else "")
         h[k2] = h.get(k2, 0) + 1
-    return {"requests": K, "budget_s": args.tree_budget, "histogram_by_machines": hist,
-            "rows": [{"m": m, "n": n, "status": st, "decomposed": dec, "seconds": round(sec, 4)}
-                     for m, n, st, dec, sec in rows]}
+    return {"requests": K, "budget_s": args.tree_budget, "incumbent": args.tree_ub, "ig": ig,
+            "histogram_by_machines": hist,
+            "solved": sum(r[2] == "solved" for r in rows),
+            "rows": [{"m": m, "n": n, "status": st, "decomposed": dec, "seconds": round(sec, 4), "ub": u,
+                      "best": b} for m, n, st, dec, sec, u, b in rows]}
 
 
 def leg_long(P, args):
@@ -620,6 +648,12 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=SERVE_RATE, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-budget", type=float, default=SERVE_BUDGET_S,
                     help="per-request time budget after admission (s)")
+    ap.add_argument("--tree-ub", choices=("neh", "ig"), default="neh",
+                    help="incumbent of the tree-size leg: NEH, or the device iterated greedy")
+    ap.add_argument("--ig-chains", type=int, default=64)
+    ap.add_argument("--ig-iters", type=int, default=100)
+    ap.add_argument("--serve-paths", default="pool,routed,plain",
+                    help="which of pool,routed,plain the serving leg replays")
     ap.add_argument("--autotune", action="store_true", help="re-run the pool autotuner instead of tuned.json")
     ap.add_argument("--tree-requests", type=int, default=300)
     ap.add_argument("--tree-budget", type=float, default=20.0)
