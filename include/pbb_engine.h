@@ -71,6 +71,10 @@ typedef struct pbb_config {
                                a refinement bound) for up to this many nodes,
                                then on the whole device (0: whole device at
                                once); engine-specific, no reference counterpart */
+    int team_waves;         /* explorers of that workgroup (0: the largest the
+                               kernel allows, 16 or 8; larger values are
+                               clamped to it); engine-specific, no reference
+                               counterpart */
 } pbb_config;
 
 /* Defaults of arguments.cpp:135-175 with sortedDFS forced to 0. */

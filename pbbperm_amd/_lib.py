@@ -36,7 +36,7 @@ class Config(C.Structure):
         ("single_node", C.c_int), ("sorted_dfs", C.c_int), ("nb_ivms", C.c_int),
         ("steal_margin", C.c_int), ("init_mode", C.c_int), ("init_solution", C.c_int),
         ("checkpoint_s", C.c_int), ("balance_s", C.c_int), ("ramp_nodes", C.c_int),
-        ("team_nodes", C.c_int),
+        ("team_nodes", C.c_int), ("team_waves", C.c_int),
     ]
 
     def as_dict(self):

@@ -378,6 +378,7 @@ extern "C" int pbb_create(const int32_t *ptm, int n, int m, const pbb_config *cf
         const bool lb2 = cfg->bound_mode != 0 && cfg->refine == 1;
         const int ref = cfg->bound_mode == 0 ? 0 : (cfg->refine == 1 ? 2 : 1);
         int w = std::min(e->E, team_waves_max(ref));
+        if (cfg->team_waves > 0) w = std::min(w, cfg->team_waves);
         while (w > 0 && team_lds_bytes(e->D.I, lb2, w) > 160 * 1024) w--;
         e->team_E = w;
     }

@@ -58,6 +58,7 @@ extern "C" int pbb_config_default(pbb_config *c)
     c->balance_s = 1;
     c->ramp_nodes = 0;
     c->team_nodes = 8192;
+    c->team_waves = 0;
     return PBB_OK;
 }
 
@@ -78,6 +79,7 @@ int pbb::validate(const pbb_config *c)
     if (c->steal_margin < 0) return fail(PBB_E_ARG, "steal margin %d is negative", c->steal_margin);
     if (c->ramp_nodes < 0) return fail(PBB_E_ARG, "ramp %d is negative", c->ramp_nodes);
     if (c->team_nodes < 0) return fail(PBB_E_ARG, "team node budget %d is negative", c->team_nodes);
+    if (c->team_waves < 0) return fail(PBB_E_ARG, "team size %d is negative", c->team_waves);
     // earlyStopJohnson (bound_fsp_strong.cpp:114-123, :150-154): the Johnson
     // bound stops at the first pair whose value exceeds the incumbent and
     // reorders its pairs by past success, state of each bound object.  With
@@ -184,6 +186,7 @@ extern "C" int pbb_config_load_ini(const char *path, pbb_config *c, char *proble
     c->steal_margin = (int)ini_int(kv, "gpu.stealmargin", 6);
     c->ramp_nodes = (int)ini_int(kv, "gpu.rampnodes", 0);
     c->team_nodes = (int)ini_int(kv, "gpu.teamnodes", 8192);
+    c->team_waves = (int)ini_int(kv, "gpu.teamwaves", 0);
     return validate(c);
 }
 

@@ -68,11 +68,11 @@ def test_ini_surface(tmp_path):
         P.load_config(str(bad))
     # engine-specific keys: the team phase and the donation ramp (defaults
     # 8192 and 0), negative values refused
-    assert d["team_nodes"] == 8192 and d["ramp_nodes"] == 0
-    bad.write_text("[bb]\nsortedDFS = 0\n[gpu]\nteamNodes = 0\nrampNodes = 32\n")
+    assert d["team_nodes"] == 8192 and d["ramp_nodes"] == 0 and d["team_waves"] == 0
+    bad.write_text("[bb]\nsortedDFS = 0\n[gpu]\nteamNodes = 0\nrampNodes = 32\nteamWaves = 4\n")
     cfg2, _, _ = P.load_config(str(bad))
-    assert cfg2.team_nodes == 0 and cfg2.ramp_nodes == 32
-    for key in ("teamNodes", "rampNodes"):
+    assert cfg2.team_nodes == 0 and cfg2.ramp_nodes == 32 and cfg2.team_waves == 4
+    for key in ("teamNodes", "rampNodes", "teamWaves"):
         bad.write_text(f"[bb]\nsortedDFS = 0\n[gpu]\n{key} = -1\n")
         with pytest.raises(P.PbbError):
             P.load_config(str(bad))
