@@ -15,11 +15,16 @@ enum { PBB_DEV_ERR_RING = 1 };
 
 #define PBB_SPIN_LIMIT (1 << 24)
 
-// A busy explorer reads the idle ring every PBB_POLL_EVERY decomposed nodes
-// while it was empty at the last read, and at every node while explorers
-// wait in it (power of two).  Measured: the 50x20 explorer is the same at 1,
-// 4 and 16 (1500-1530 M nodes/s); the 512-instance pool batch takes 500-518
-// ms at 1, 459-490 at 4 and 433-438 at 16.
+// A busy explorer of a device-wide launch reads the idle ring every
+// PBB_POLL_EVERY decomposed nodes (power of two), whether or not explorers
+// wait in it.  Polling at every node while some waited (the earlier policy)
+// had every busy explorer of a small tree race for the same ring ticket at
+// every node: on a 5.0 M-node 19x10 LB2 tree with a frozen incumbent the
+// plain engine took 94 / 275 / 354 ms at 256 / 1024 / 3072 explorers; polling
+// every 4 / 8 / 16 nodes instead, 89 / 81 / 95, 88 / 52 / 61 and 89 / 47 / 53
+// ms; the 512-instance batch at 1024 explorers per instance 599-929 ms
+// before, 178-189 ms at 16; the 50x20 explorer 1984 M nodes/s before,
+// 2036 M at 16.
 #ifndef PBB_POLL_EVERY
 #define PBB_POLL_EVERY 16
 #endif

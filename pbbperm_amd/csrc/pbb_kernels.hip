@@ -97,8 +97,14 @@ __device__ __forceinline__ void explore_loop(const KArgs &a, const Lds &lds, int
             }
         }
         // idle-ring poll issued before the node's work so its latency is
-        // hidden; none while the ramp holds this explorer's work back
-        const bool poll = X.since >= a.ramp && (hungry || (iter & (PBB_POLL_EVERY - 1)) == 0u);
+        // hidden; none while the ramp holds this explorer's work back.  In
+        // the workgroup team (at most 16 explorers, workgroup-scope atomics)
+        // a donor that saw explorers waiting polls again at the next node; at
+        // device scope every busy explorer racing for the same ring ticket at
+        // every node made the ring words a hot spot, so there it polls every
+        // PBB_POLL_EVERY nodes only.
+        constexpr bool team = SC == __HIP_MEMORY_SCOPE_WORKGROUP;
+        const bool poll = X.since >= a.ramp && ((team && hungry) || (iter & (PBB_POLL_EVERY - 1)) == 0u);
         unsigned rh = 0, rt = 0;
         if (poll) {
             rh = ld_rlx<SC>(&a.ctl->idle_head);

@@ -729,7 +729,7 @@ extern "C" int pbb_pool_autotune(const pbb_config *cfg, const pbb_pool_params *b
         return fail(PBB_E_ARG, "bad autotune arguments");
     // most-promising first, so the time cap tightens early
     const int Ws[] = {4, 2, 1};
-    const int epis[] = {256, 1024, 64};
+    const int epis[] = {256, 1024, 3072, 64};
     const int margins[] = {2, 4, 8};
     std::vector<pbb_result> res(count);
     double best_ms = 1e300;

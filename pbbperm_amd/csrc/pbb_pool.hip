@@ -238,6 +238,11 @@ static __device__ __forceinline__ int pool_pick(const PoolArgs &p, int W, unsign
                 // pairs with the host's release of the tail read above
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
                 *slot = uni(sys_ld_rlx(&p.q->ids[head & (unsigned)(p.qcap - 1)]));
+                // the slot counts as occupied from here on: a group that finds
+                // the queue closed and drained waits while a request is
+                // still being admitted, instead of leaving the launch
+                if (lane == 0)
+                    __hip_atomic_store(&slot_hdr(p, *slot)->occ, W, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 return ROLE_OWNER;
             }
             continue;
@@ -255,7 +260,7 @@ static __device__ __forceinline__ int pool_pick(const PoolArgs &p, int W, unsign
             const bool stopping = live && ld_rlx(&slot_hdr(p, s)->c.stop) != 0;
             // more busy than idle members: work is there to be split
             const bool cand = live && !stopping && mem + W <= p.epi && 2 * wc > mem;
-            running = running || ballot(live) != 0ull;
+            running = running || ballot(o != 0) != 0ull;
             const int sc = cand ? 2 * wc - mem : -1;
             const int mx = wave_max(sc);
             if (mx > best_score) {
@@ -441,7 +446,6 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES(M)) void k_pool(KArgs a, Po
                 X.register_idle();
             }
             unsigned iter = 0;
-            bool hungry = true;
             for (;;) {
                 if (st == ST_IDLE) {
                     bool leave = false;
@@ -465,7 +469,7 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES(M)) void k_pool(KArgs a, Po
                     }
                     if (leave) break;
                 }
-                const bool poll = hungry || (iter & (PBB_POLL_EVERY - 1)) == 0u;
+                const bool poll = (iter & (PBB_POLL_EVERY - 1)) == 0u;
                 unsigned rh = 0, rt = 0;
                 if (poll) {
                     rh = ld_rlx(&S->c.idle_head);
@@ -489,7 +493,7 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES(M)) void k_pool(KArgs a, Po
                         continue;
                     }
                 }
-                if (poll) hungry = X.try_donate(rh, rt);
+                if (poll) X.try_donate(rh, rt);
             }
             X.flush_counters();
         }

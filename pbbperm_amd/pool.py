@@ -253,7 +253,7 @@ def plain_replay(ptms: Sequence[np.ndarray], ubs: Sequence[int], arrivals: Seque
 
 # searched most-promising first, so the time cap tightens early
 GRID_W = (4, 2, 1)
-GRID_EPI = (256, 1024, 64)
+GRID_EPI = (256, 1024, 3072, 64)
 GRID_MARGIN = (2, 4, 8)
 
 

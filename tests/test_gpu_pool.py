@@ -572,3 +572,29 @@ def test_pool_computes_neh_at_admission(oracle_mod):
     for p, r, u in zip(ptms[:8], plain, opts[:8]):
         assert r.status == "solved" and r.best == u and r.improved
         _check_schedule(P, p, r)
+
+
+def test_single_request_pool_recruits_every_group():
+    """A batch of one request: while its owner group admits it the other
+    groups find the queue closed and drained, and must wait for the slot to
+    run rather than leave the launch (before they did, and the 19x10 tree
+    below took 5.5 s on the owner's 4 explorers).  The counts with the
+    incumbent frozen at the optimum equal the plain engine's, and the solve
+    takes a small fraction of that time."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    p = PP.batch_stream(512, 100)[311]
+    assert p.shape == (10, 19)
+    opt = 1434
+    cfg = PP.workload_config("batched")
+    with P.GpuBB(p, PP.workload_config("batched", nb_ivms=1024), ub=opt) as eng:
+        eng.solve(time_budget=BUDGET)
+        want = [eng.stats().decomposed, eng.stats().leaves]
+    with PP.Pool(cfg, max_jobs=22, machines=10, slots=1, waves_per_group=4, explorers_per_instance=1024,
+                 steal_margin=2) as pool:
+        pool.solve([p], [opt], time_budget=BUDGET)
+        r, ms = pool.solve([p], [opt], time_budget=BUDGET)
+    print(f"19x10 frozen at {opt}: {r[0].decomposed} nodes in {ms:.1f} ms (plain {want})", flush=True)
+    assert r[0].status == "solved" and r[0].best == opt
+    assert [r[0].decomposed, r[0].leaves] == want
+    assert ms < 1500.0
