@@ -6,8 +6,9 @@ For each workload the autotuner (pbbperm_amd.pool.autotune) scores every
 (waves per workgroup, explorers per instance, steal margin) candidate on a
 tuning sample of the workload's distribution with seeds disjoint from the
 measured stream -- the batch's makespan for "batched"; for "serving", the
-mean latency of a Poisson replay of the sample at 0.8 x the plain path's
-closed-loop rate -- and the best is written to PATH
+mean latency of a Poisson replay of the sample at the serving leg's rate (20
+requests/s) and per-request budget (2 s, stopped requests counted at their
+latency) -- and the best is written to PATH
 (default pbbperm_amd/tuned.json) with the trial table.  bench.py reads that
 file.
 """
