@@ -409,6 +409,18 @@ def leg_serving(P, PP, args):
     cfg = PP.workload_config("serving")
     ptms = PP.serving_stream(N, 10000)
     ubs = [P.neh(p)[0] for p in ptms]
+    ig = None
+    if args.serve_ub == "ig":
+        # incumbents from the device iterated greedy, computed for the whole
+        # stream in one launch before the replay (its time is reported, not
+        # added to any request's latency)
+        PP.ig_device(ptms[:1], chains=args.ig_chains, iterations=1, seed=1)
+        t = time.perf_counter()
+        _, costs = PP.ig_device(ptms, chains=args.ig_chains, iterations=args.ig_iters, seed=1)
+        ig = {"chains": args.ig_chains, "iterations": args.ig_iters,
+              "seconds_all_requests": round(time.perf_counter() - t, 4),
+              "below_neh": int(sum(int(c) < u for c, u in zip(costs, ubs)))}
+        ubs = [int(c) for c in costs]
     best, src = _params(PP, "serving", args)
     best.request_budget_s = args.serve_budget
     pcfg = PP.workload_config("serving", nb_ivms=best.explorers_per_instance)
@@ -454,9 +466,10 @@ def leg_serving(P, PP, args):
 
     out = {
         "workload": f"{N} Poisson requests at {rate:g}/s, seeds 10000-{10000 + N - 1}, n~U[12,30], "
-                    "m~U{5,10,20}, LB1 + Johnson LB2 (all pairs), MinMin, NEH incumbents, per-request budget "
+                    f"m~U{{5,10,20}}, LB1 + Johnson LB2 (all pairs), MinMin, {'IG' if ig else 'NEH'} incumbents, per-request budget "
                     f"{args.serve_budget:g} s after admission (over-budget requests are stopped and counted late)",
         "requests": N, "rate_per_s": rate, "request_budget_s": args.serve_budget, "paths": sorted(paths),
+        "incumbent": args.serve_ub, "ig": ig,
     }
     if res is not None:
         by_m = {}
@@ -650,6 +663,8 @@ def main():
                     help="per-request time budget after admission (s)")
     ap.add_argument("--tree-ub", choices=("neh", "ig"), default="neh",
                     help="incumbent of the tree-size leg: NEH, or the device iterated greedy")
+    ap.add_argument("--serve-ub", choices=("neh", "ig"), default="neh",
+                    help="incumbents of the serving stream: NEH, or the device iterated greedy (precomputed)")
     ap.add_argument("--ig-chains", type=int, default=64)
     ap.add_argument("--ig-iters", type=int, default=100)
     ap.add_argument("--serve-paths", default="pool,routed,plain",
