@@ -286,7 +286,6 @@ struct PoolArgs {
     unsigned *q_head;             // consumer ticket (device memory)
     Ctl *run;                     // stop / error of the launch
     unsigned long long req_ticks; // per-request budget after admission (wall_clock64 ticks), 0 = none
-    int tail_cap;                 // explorer cap per instance once the queue is closed and drained
 };
 
 __device__ __host__ inline size_t pool_lds_bytes(int nmax, int M, int pmax, bool jt_lds, int W)

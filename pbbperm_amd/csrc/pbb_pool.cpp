@@ -113,10 +113,6 @@ static KArgs pool_kargs(const pbb_pool *p, double budget_s)
     return a;
 }
 
-#ifndef PBB_POOL_TAIL_JOIN
-#define PBB_POOL_TAIL_JOIN 0
-#endif
-
 static PoolArgs pool_pargs(const pbb_pool *p, bool serve)
 {
     PoolArgs q{};
@@ -128,7 +124,6 @@ static PoolArgs pool_pargs(const pbb_pool *p, bool serve)
     q.nmax = p->nmax;
     q.pmax = p->pmax;
     q.ring_cap = p->ring_cap;
-    q.tail_cap = PBB_POOL_TAIL_JOIN ? std::max(p->epi, std::min(p->E, p->ring_cap / 2)) : p->epi;
     q.jt_lds = p->jt_lds;
     q.pairs_mode = p->cfg.johnson_pairs;
     q.qcap = p->qcap;
@@ -261,7 +256,7 @@ extern "C" int pbb_pool_create(const pbb_config *cfg, const pbb_pool_params *prm
     }
     p->E = p->groups * p->W;
     p->epi = std::min(p->epi, p->E);
-    p->ring_cap = next_pow2(std::max(64, 2 * (PBB_POOL_TAIL_JOIN ? p->E : p->epi)));
+    p->ring_cap = next_pow2(std::max(64, 2 * p->epi));
     p->lay = pool_layout(p->nmax, p->M, p->pmax, p->ring_cap);
     const size_t S = (size_t)prm->slots, E = (size_t)p->E;
     HTRY(hipMalloc(&p->arena, S * p->lay.stride));

@@ -247,8 +247,6 @@ static __device__ __forceinline__ int pool_pick(const PoolArgs &p, int W, unsign
             }
             continue;
         }
-        const int closed = uni(sys_ld_rlx(&p.q->closed));
-        const int cap = closed && (int)head >= tail ? p.tail_cap : p.epi;
         int best_score = -1, best_slot = -1, best_occ = 0;
         bool running = false;
         for (int base = 0; base < p.nslots; base += PBB_WAVE) {
@@ -261,7 +259,7 @@ static __device__ __forceinline__ int pool_pick(const PoolArgs &p, int W, unsign
             // a slot whose request ran out of time is draining: do not join it
             const bool stopping = live && ld_rlx(&slot_hdr(p, s)->c.stop) != 0;
             // more busy than idle members: work is there to be split
-            const bool cand = live && !stopping && mem + W <= cap && 2 * wc > mem;
+            const bool cand = live && !stopping && mem + W <= p.epi && 2 * wc > mem;
             running = running || ballot(o != 0) != 0ull;
             const int sc = cand ? 2 * wc - mem : -1;
             const int mx = wave_max(sc);
@@ -279,6 +277,7 @@ static __device__ __forceinline__ int pool_pick(const PoolArgs &p, int W, unsign
             }
             continue;
         }
+        const int closed = uni(sys_ld_rlx(&p.q->closed));
         if (closed && (int)head >= tail && !running) return ROLE_EXIT;
         __builtin_amdgcn_s_sleep(32);
     }
