@@ -395,12 +395,15 @@ WORKLOADS = {
 # The serving stream split over two pools resident together (replay_routed):
 # 5- and 10-machine requests on a 10-machine pool, 20-machine requests on a
 # 20-machine pool, both for n <= 30.  The 20-machine pool starts first (its
-# waves need the most registers) and keeps 192 groups; the 10-machine pool
-# takes 512 groups beside it (a split first measured on a stream whose
-# 20-machine requests had n <= 16: 576+128 groups gave p50 / mean latency
-# 13.5 / 52.2 ms, 512+192 11.2 / 49.5, 384+256 21.4 / 157).
-SERVING_ROUTED = (dict(machines=10, max_jobs=30, groups=512),
-                  dict(machines=20, max_jobs=30, groups=192))
+# waves need the most registers).  Measured on the 600-request serving stream
+# (20 req/s, 2 s budget per request, the stored serving tuning), groups of
+# the 10-machine + 20-machine pools: 512+192 gave 165 late, p50 5.7 ms, p99
+# 1.86 s; 384+256 156, 4.7 ms, 1.87 s; 256+320 145-149, 4.1-4.4 ms,
+# 1.52-1.83 s; 192+352 146, 3.9 ms, 1.76 s; 128+384 137, 4.0 ms, 1.46 s;
+# 128+416 133, 4.0 ms, 1.89 s.  Most of the work is in 20-machine requests,
+# and a 10-machine pool of 128 groups still serves the rest within a few ms.
+SERVING_ROUTED = (dict(machines=10, max_jobs=30, groups=128),
+                  dict(machines=20, max_jobs=30, groups=384))
 
 
 def serving_routed_params(base: PoolParams) -> List[PoolParams]:
