@@ -345,6 +345,24 @@ def leg_batched(P, PP, args):
             hd, hr = pool.batch_histograms(len(ptms))
             opss.append(pool_ops(ptms, hd, hr, lambda m: m * (m - 1) // 2))
         info = pool.info()
+    # the same pool without the IG helper wave: the plain path below has only
+    # the NEH incumbents, so this is the like-for-like comparison
+    noig = None
+    if best.ig_iterations > 0:
+        import ctypes
+        prm0 = PP.PoolParams()
+        ctypes.memmove(ctypes.byref(prm0), ctypes.byref(best), ctypes.sizeof(PP.PoolParams))
+        prm0.ig_iterations = 0
+        with PP.Pool(cfg, params=prm0) as pool0:
+            pool0.solve(ptms, ubs, time_budget=120.0)
+            w0, r0 = [], None
+            for _ in range(3):
+                t = time.perf_counter()
+                r0, _ = pool0.solve(ptms, ubs, time_budget=120.0)
+                w0.append(time.perf_counter() - t)
+        noig = {"wall_ms": min(w0) * 1e3, "solves_per_s": len(ptms) / min(w0),
+                "decomposed": int(sum(r.decomposed for r in r0)),
+                "optima_equal": all(a.best == b.best for a, b in zip(res, r0))}
     note(f"batched: pool {min(walls) * 1e3:.1f} ms; plain path")
     # the plain path gets the same per-instance explorer count
     pcfg = PP.workload_config("batched", nb_ivms=best.explorers_per_instance)
@@ -380,15 +398,18 @@ def leg_batched(P, PP, args):
                      "traffic": BATCH_HBM_BYTES, "traffic_unit": "HBM bytes per launch (rocprofv3 "
                      "FETCH_SIZE/WRITE_SIZE, calibrated)" if BATCH_HBM_BYTES else None},
         "workload": "512 instances, seeds 100-611, n~U[10,22], m~U{5,10}, LB1 + Johnson LB2 (all pairs), "
-                    "MinMin branching, NEH incumbents, solved to optimality in one graph launch",
+                    "MinMin branching, NEH incumbents (the pool adds an IG helper wave per instance, "
+                    f"{best.ig_iterations} rounds at most), solved to optimality in one graph launch",
         "solves_per_s": len(ptms) / wall, "wall_ms": wall * 1e3, "graph_ms": min(kms),
         "decomposed": nodes, "nodes_per_s": nodes / wall, "tail": tail,
         "plain_solves_per_s": len(ptms) / pwall, "plain_wall_ms": pwall * 1e3,
         "plain_explorers_per_instance": best.explorers_per_instance,
         "speedup_vs_plain": pwall / wall,
+        "ig_iterations": best.ig_iterations, "without_ig_helper": noig,
+        "speedup_vs_plain_without_ig_helper": (pwall * 1e3 / noig["wall_ms"]) if noig else None,
         "all_solved": ok, "optima_equal_plain": same, "optima_equal_reference": ref_match,
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
-                  "steal_margin": best.steal_margin, "source": src},
+                  "steal_margin": best.steal_margin, "ig_iterations": best.ig_iterations, "source": src},
         "explorers": info["explorers"],
     }
     if not args.no_cpu:
@@ -423,6 +444,8 @@ def leg_serving(P, PP, args):
         ubs = [int(c) for c in costs]
     best, src = _params(PP, "serving", args)
     best.request_budget_s = args.serve_budget
+    if args.serve_ig is not None:
+        best.ig_iterations = args.serve_ig
     pcfg = PP.workload_config("serving", nb_ivms=best.explorers_per_instance)
     rate = args.serve_rate
     arrivals = PP.poisson_arrivals(N, rate, seed=7)
@@ -467,7 +490,9 @@ def leg_serving(P, PP, args):
     out = {
         "workload": f"{N} Poisson requests at {rate:g}/s, seeds 10000-{10000 + N - 1}, n~U[12,30], "
                     f"m~U{{5,10,20}}, LB1 + Johnson LB2 (all pairs), MinMin, {'IG' if ig else 'NEH'} incumbents, per-request budget "
-                    f"{args.serve_budget:g} s after admission (over-budget requests are stopped and counted late)",
+                    f"{args.serve_budget:g} s after admission (over-budget requests are stopped and counted late); "
+                    f"the pools add an IG helper wave per request ({best.ig_iterations} rounds at most), "
+                    "the plain path has the incumbents alone",
         "requests": N, "rate_per_s": rate, "request_budget_s": args.serve_budget, "paths": sorted(paths),
         "incumbent": args.serve_ub, "ig": ig,
     }
@@ -490,7 +515,7 @@ def leg_serving(P, PP, args):
         "optima_equal_plain": agree(res, plain),
         "optima_vs_reference": ref_check,
         "tuned": {"waves_per_group": best.waves_per_group, "explorers_per_instance": best.explorers_per_instance,
-                  "steal_margin": best.steal_margin, "source": src},
+                  "steal_margin": best.steal_margin, "ig_iterations": best.ig_iterations, "source": src},
     })
     return out
 
@@ -663,6 +688,8 @@ def main():
                     help="per-request time budget after admission (s)")
     ap.add_argument("--tree-ub", choices=("neh", "ig"), default="neh",
                     help="incumbent of the tree-size leg: NEH, or the device iterated greedy")
+    ap.add_argument("--serve-ig", type=int, default=None,
+                    help="IG helper rounds per request in the serving pools (default: the stored tuning's)")
     ap.add_argument("--serve-ub", choices=("neh", "ig"), default="neh",
                     help="incumbents of the serving stream: NEH, or the device iterated greedy (precomputed)")
     ap.add_argument("--ig-chains", type=int, default=64)

@@ -329,6 +329,11 @@ typedef struct pbb_pool_params {
     double request_budget_s;    /* per-instance time after admission (0 = none): an instance
                                    still running then is stopped and retires PBB_RES_ABORTED,
                                    with the best schedule found so far */
+    int ig_iterations;          /* 0 = none; else the last wave of each instance's admitting
+                                   workgroup runs one iterated-greedy chain of this many
+                                   rounds beside the search, publishing every improvement as
+                                   the instance's incumbent (needs waves_per_group >= 2);
+                                   engine-specific, no reference counterpart */
 } pbb_pool_params;
 
 typedef struct pbb_result {

@@ -59,7 +59,7 @@ class PoolParams(C.Structure):
         ("max_jobs", C.c_int), ("machines", C.c_int), ("slots", C.c_int),
         ("waves_per_group", C.c_int), ("explorers_per_instance", C.c_int),
         ("steal_margin", C.c_int), ("groups", C.c_int), ("hbm_budget_gb", C.c_double),
-        ("request_budget_s", C.c_double),
+        ("request_budget_s", C.c_double), ("ig_iterations", C.c_int),
     ]
 
     def as_dict(self):

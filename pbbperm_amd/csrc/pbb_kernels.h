@@ -286,6 +286,7 @@ struct PoolArgs {
     unsigned *q_head;             // consumer ticket (device memory)
     Ctl *run;                     // stop / error of the launch
     unsigned long long req_ticks; // per-request budget after admission (wall_clock64 ticks), 0 = none
+    int ig_iters;                 // IG helper rounds per instance (0 = no helper wave)
 };
 
 __device__ __host__ inline size_t pool_lds_bytes(int nmax, int M, int pmax, bool jt_lds, int W)
@@ -313,6 +314,14 @@ hipError_t launch_neh_m(const int *ptms, const long long *off, const int *ns, co
 // iterated greedy (pbb_ig.hip, one translation unit per M): acceptance
 // thresholds per instance for cost deltas 0..IG_THR-1
 #define IG_THR 256
+// LDS of the pool's IG helper wave, after the pool's own LDS: acceptance
+// thresholds (IG_THR x u64), job totals (64), heads and tails (2 x M x 64)
+// and scratch (64 ints)
+__device__ __host__ inline size_t pool_ig_bytes(int M) { return (size_t)IG_THR * 8 + ((size_t)2 * M * 64 + 128) * 4; }
+__device__ __host__ inline size_t pool_lds_total(const PoolArgs &p, int M)
+{
+    return pool_lds_bytes(p.nmax, M, p.pmax, p.jt_lds != 0, p.W) + (p.ig_iters > 0 ? pool_ig_bytes(M) : 0);
+}
 template <int M>
 hipError_t launch_ig_m(const int *ptms, const long long *off, const int *ns, const int *ms, int count, int chains,
                        int iters, unsigned long long seed, const unsigned long long *thr, int *perms, int *costs,

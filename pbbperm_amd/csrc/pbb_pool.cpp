@@ -43,6 +43,7 @@ struct pbb_pool {
     pbb_config cfg;
     pbb_pool_params prm;
     int M = 0, nmax = 0, pmax = 0, ring_cap = 0, jt_lds = 0, W = 0, epi = 0, margin = 0;
+    int ig = 0;  // IG helper rounds per instance (0 = none)
     int groups = 0, E = 0, qcap = 0;
     size_t lds = 0;
     PoolLayout lay{};
@@ -124,6 +125,7 @@ static PoolArgs pool_pargs(const pbb_pool *p, bool serve)
     q.nmax = p->nmax;
     q.pmax = p->pmax;
     q.ring_cap = p->ring_cap;
+    q.ig_iters = p->ig;
     q.jt_lds = p->jt_lds;
     q.pairs_mode = p->cfg.johnson_pairs;
     q.qcap = p->qcap;
@@ -150,6 +152,7 @@ extern "C" int pbb_pool_params_default(pbb_pool_params *prm)
     prm->groups = 0;
     prm->hbm_budget_gb = 0.0;
     prm->request_budget_s = 0.0;
+    prm->ig_iterations = 0;
     return PBB_OK;
 }
 
@@ -204,7 +207,8 @@ extern "C" int pbb_pool_create(const pbb_config *cfg, const pbb_pool_params *prm
         return fail(PBB_E_ARG, "waves_per_group must be 1, 2 or 4");
     if (prm->explorers_per_instance < prm->waves_per_group)
         return fail(PBB_E_ARG, "explorers_per_instance below waves_per_group");
-    if (prm->steal_margin < 0 || prm->groups < 0 || prm->hbm_budget_gb < 0 || prm->request_budget_s < 0)
+    if (prm->steal_margin < 0 || prm->groups < 0 || prm->hbm_budget_gb < 0 || prm->request_budget_s < 0 ||
+        prm->ig_iterations < 0)
         return fail(PBB_E_ARG, "negative pool parameter");
     if ((rc = check_device())) return rc;
 
@@ -219,7 +223,8 @@ extern "C" int pbb_pool_create(const pbb_config *cfg, const pbb_pool_params *prm
     const bool lb2 = cfg->bound_mode != 0 && cfg->refine == 1;
     p->pmax = lb2 ? (cfg->johnson_pairs == 0 ? p->M * (p->M - 1) / 2 : p->M - 1) : 0;
     p->jt_lds = (lb2 && jtab_in_lds(p->nmax, p->pmax)) ? 1 : 0;
-    p->lds = pool_lds_bytes(p->nmax, p->M, p->pmax, p->jt_lds != 0, p->W);
+    p->ig = p->W >= 2 ? prm->ig_iterations : 0;
+    p->lds = pool_lds_bytes(p->nmax, p->M, p->pmax, p->jt_lds != 0, p->W) + (p->ig ? pool_ig_bytes(p->M) : 0);
     p->qcap = next_pow2(std::max(64, prm->slots));
 #define TRY(x)              \
     do {                    \

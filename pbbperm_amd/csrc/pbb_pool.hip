@@ -30,7 +30,7 @@
 #include <algorithm>
 #include <cstddef>
 
-#include "pbb_explorer.h"
+#include "pbb_ig.h"
 
 namespace pbb {
 
@@ -192,7 +192,9 @@ static __device__ __forceinline__ bool pool_admit(const PoolArgs &p, int s, bool
     // control block, incumbent, idle ring
     int *ci = (int *)&S->c;
     const int wc = (int)(offsetof(Ctl, work_count) / 4);
-    for (int i = tid; i < (int)(sizeof(Ctl) / 4); i += nt) ci[i] = (i == wc) ? 1 : 0;
+    // explorer 0 holds the full interval; an IG helper wave holds one more unit
+    const int held = p.ig_iters > 0 ? 2 : 1;
+    for (int i = tid; i < (int)(sizeof(Ctl) / 4); i += nt) ci[i] = (i == wc) ? held : 0;
     unsigned long long *rs = (unsigned long long *)(b + p.lay.ring);
     for (int i = tid; i < p.ring_cap; i += nt) rs[i] = (unsigned long long)(unsigned)i << 32;
     if (tid == 0) {
@@ -337,6 +339,59 @@ static __device__ __forceinline__ void pool_leave(const PoolArgs &p, int s, int 
     if (lane == 0) sys_st_rel(&R->tag, tag);
 }
 
+// ------------------------------------------------------------ IG helper
+// The last wave of an instance's admitting workgroup runs one chain of
+// IG.cpp runIG (pbb_ig.h) beside the search and publishes every improvement
+// as the instance's incumbent: its schedule into this explorer's best_perm
+// row, then the incumbent word (the explorers reload it every
+// PBB_BEST_EVERY nodes and prune with it).  The wave holds one unit of the
+// slot's work counter from admission on, so the instance cannot retire while
+// the chain may still write its schedule; the chain stops once it is the
+// only holder left (the search is over), on the slot's or the launch's stop
+// word, or at the deadline, and the wave then gives the unit up like an
+// explorer giving up its interval (the last one out records the result).
+// Acceptance temperature as the reference's (IG.cpp:319-338): 0.2 x the
+// mean processing time / 10, thresholds floor(2^32 exp(-delta / T)).
+// It runs before the wave builds its explorer, so the chain's registers and
+// the explorer's are never live together.
+template <int M>
+__device__ __forceinline__ void pool_ig_helper(const Lds &lds, int n, int e, const KArgs &a, const PoolArgs &p,
+                                               PoolSlot *S, int slot, char *smem, unsigned long long t_end)
+{
+    const int lane = lane_id();
+    unsigned long long *thr =
+        (unsigned long long *)(smem + pool_lds_bytes(p.nmax, M, p.pmax, p.jt_lds != 0, p.W));
+    int *spt = (int *)(thr + IG_THR);
+    int *hd = spt + 64, *tl = hd + M * 64, *scr = tl + M * 64;
+    int tot = 0;
+    if (lane < n)
+        for (int q = 0; q < M; q++) tot += lds.pm[q * PM_STRIDE + lane];
+    spt[lane] = tot;
+    const int m = uni(ld_rlx(&S->m));
+    const double T = 0.2 * ((double)wave_sum(tot) / ((double)n * m)) / 10.0;
+    for (int d = lane; d < IG_THR; d += PBB_WAVE) {
+        const double pr = T > 0.0 ? exp(-(double)d / T) : (d == 0 ? 1.0 : 0.0);
+        const double v = floor(pr * 4294967296.0);
+        thr[d] = v >= 4294967296.0 ? (1ull << 32) : (unsigned long long)v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    IgWave<M> G{lds.pm, spt, hd, tl, scr, n, lane};
+    uint64_t rng = 0x243F6A8885A308D3ull ^ (((uint64_t)(uint32_t)ld_rlx(&S->tag) << 32) | (uint32_t)slot);
+    rng = ig_next(rng);
+    const auto stop = [&] {
+        return uni(ld_rlx(&S->c.work_count)) <= 1 || uni(ld_rlx(&S->c.stop)) != 0 ||
+               uni(ld_rlx(&p.run->stop)) != 0 || (t_end && uni64(wall_clock64()) >= t_end);
+    };
+    const auto publish = [&](int cost, int seq) {
+        const int cur = uni((int)(unsigned)(ld_rlx(&S->best) >> 32));
+        if (cost >= cur) return;
+        if (lane < n) a.best_perm[(size_t)e * 64 + lane] = seq;
+        w_min64(&S->best, ((unsigned long long)(unsigned)cost << 32) | (unsigned)e);
+    };
+    int best = 0;
+    ig_chain<M>(G, rng, thr, p.ig_iters, &best, stop, publish);
+}
+
 // ----------------------------------------------------------------- kernel
 // Minimum resident waves per SIMD the compiler must allow (register
 // budget): 3 up to 10 machines (168 VGPRs: the 10-machine LB2 pool would
@@ -407,21 +462,31 @@ __global__ __launch_bounds__(256, PBB_POOL_MIN_WAVES(M)) void k_pool(KArgs a, Po
                 __syncthreads();
             }
             const int ub = uni(ld_rlx(&S->ub));
-            Explorer<M, REF> X(a, lds, e, slot_ref(p, slot, S, m, npairs, ub), p.run, n, I.mp);
             // The request's own deadline (admission + its budget) and the
             // launch's, whichever comes first, raise the slot's stop word: its
             // explorers then give their intervals up as if exhausted, so the
             // slot drains through the ordinary release path and every member
             // leaves together once it is done (none leaves while a donor might
             // still hand it work).
-            X.t_end = t_end;
+            unsigned long long slot_end = t_end;
             if (p.req_ticks) {
                 const unsigned long long dl = uni64(ld_rlx(&S->t_admit)) + p.req_ticks;
-                X.t_end = t_end ? (t_end < dl ? t_end : dl) : dl;
+                slot_end = t_end ? (t_end < dl ? t_end : dl) : dl;
             }
+            // the IG helper wave meets the owner's barrier below, then runs its
+            // chain before it builds its explorer
+            const bool igw = role == ROLE_OWNER && p.ig_iters > 0 && wave == W - 1;
+            if (igw) {
+                __syncthreads();
+                pool_ig_helper<M>(lds, n, e, a, p, S, slot, smem, slot_end);
+            }
+            Explorer<M, REF> X(a, lds, e, slot_ref(p, slot, S, m, npairs, ub), p.run, n, I.mp);
+            X.t_end = slot_end;
             X.local_best = uni((int)(unsigned)(ld_rlx(&S->best) >> 32));
             int st = ST_IDLE;
-            if (role == ROLE_OWNER) {
+            if (igw) {
+                if (X.go_idle()) pool_finish(X, a, S);
+            } else if (role == ROLE_OWNER) {
                 if (wave == 0) {
                     // boundRoot: the identity order bounded with LB1 and pruned
                     // against the initial UB (not counted, as in the engine)
@@ -574,7 +639,7 @@ static const void *pool_kernel(const KArgs &a)
 template <int M>
 hipError_t launch_pool_m(const KArgs &a, const PoolArgs &p, int groups, int init_count, hipStream_t s)
 {
-    const size_t lds = pool_lds_bytes(p.nmax, M, p.pmax, p.jt_lds != 0, p.W);
+    const size_t lds = pool_lds_total(p, M);
     hipLaunchKernelGGL(k_pool_init<M>, dim3(std::max(1, std::min(1024, (std::max(p.nslots, init_count) + 255) / 256))),
                        dim3(256), 0, s, p, init_count);
     hipError_t e = hipGetLastError();
@@ -594,7 +659,7 @@ hipError_t launch_pool_m(const KArgs &a, const PoolArgs &p, int groups, int init
 template <int M>
 hipError_t pool_occupancy_m(const KArgs &a, const PoolArgs &p, int *groups_per_cu)
 {
-    const size_t lds = pool_lds_bytes(p.nmax, M, p.pmax, p.jt_lds != 0, p.W);
+    const size_t lds = pool_lds_total(p, M);
     if (lds > 160 * 1024) {
         *groups_per_cu = 0;
         return hipSuccess;

@@ -265,6 +265,44 @@ def test_serving_wide_stream_matches_reference():
           f"served in {wall:.2f} s (p99 {np.percentile(lat, 99):.1f} ms)")
 
 
+def test_ig_helper_wave_keeps_every_optimum(oracle_mod):
+    """Pools whose admitting groups run an iterated-greedy helper wave beside
+    the search (ig_iterations > 0): the wide serving stream from the NEH
+    incumbent gives the reference's optimum for every request, with a
+    schedule that realises it; frozen at the optimum (nothing for the chain
+    to improve) every node count is still the reference's; a batch of small
+    instances, where the helper often outlives the search and records the
+    result itself, gives the oracle's optima."""
+    import pbbperm_amd as P
+    import pbbperm_amd.pool as PP
+    O = oracle_mod
+    with open(os.path.join(HERE, "golden", "serve_wide.json")) as f:
+        fx = json.load(f)
+    sp = fx["spec"]
+    inst = fx["instances"]
+    ptms = [P.synth(d["seed"], d["n"], d["m"]) for d in inst]
+    cfg = _cfg(P, sp["bound_mode"], sp["branching"], sp["refine"], sp["pairs"])
+    arrivals = PP.poisson_arrivals(len(ptms), rate=200.0, seed=4)
+    with PP.Pool(cfg, max_jobs=30, machines=20, slots=16, waves_per_group=4, explorers_per_instance=512,
+                 ig_iterations=30) as pool:
+        imp, _, _ = pool.replay(ptms, [d["neh"] for d in inst], arrivals, time_budget=120.0)
+        frz, _, _ = pool.replay(ptms, [d["opt"] for d in inst], arrivals, time_budget=120.0)
+    for p, r, d in zip(ptms, imp, inst):
+        assert r.status == "solved" and r.best == d["opt"], d
+        _check_schedule(P, p, r)
+    for r, d in zip(frz, inst):
+        assert r.status == "solved" and _counts(r) == d["counts_frozen"], d
+    small = _small_batch(O, 48, 8800, 8, 12)
+    opts = [_opt(O, q) for q in small]
+    with PP.Pool(_cfg(P, 2, 3, 1, 0), max_jobs=12, machines=5, slots=48, waves_per_group=2,
+                 explorers_per_instance=32, ig_iterations=5) as pool:
+        for ubs in ([PP.UB_NEH] * len(small), [P.neh(q)[0] for q in small]):
+            res, _ = pool.solve(small, ubs, time_budget=BUDGET)
+            for q, r, u in zip(small, res, opts):
+                assert r.status == "solved" and r.best == u
+                _check_schedule(P, q, r)
+
+
 def test_serving_improving_ub_optimum_exact():
     import pbbperm_amd as P
     import pbbperm_amd.pool as PP
