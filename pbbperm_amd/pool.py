@@ -385,7 +385,8 @@ def poisson_arrivals(count: int, rate: float, seed: int) -> np.ndarray:
 # beside each instance's search (ig_iterations rounds at most; it stops when
 # the search does).  Measured on the 512-instance batch (tuned 4 waves, 1024
 # explorers per instance, margin 4): 180-194 ms and 66-69 M nodes without
-# it, 115-128 ms / 35 M with 20 rounds, 107-119 ms / 30 M with 100; on the
+# it, 115-128 ms / 35 M with 20 rounds, 104-119 ms / 28-30 M with 100 (300:
+# 105-128 ms, 1000: 112-122 ms, margin 8); on the
 # 600-request serving stream (single pool): p50 106 / 82 / 69 / 69 / 60 ms
 # and 133 / 117 / 109 / 108 / 106 late at 0 / 20 / 100 / 300 / 1000 rounds.
 _LB12 = dict(bound_mode=2, branching=3, refine=1, johnson_pairs=0, early_stop_johnson=0)
