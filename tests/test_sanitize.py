@@ -3,7 +3,7 @@ VRF instance reader, the checkpoint text parser and formatter, and the
 factoradic <-> decimal codec, built with g++ -fsanitize=address,undefined
 from the library's device-free host sources (pbb_io.cpp, pbb_checkpoint.cpp)
 and driven by tests/sanitize/fuzz_parsers.cpp over seed files and seeded
-mutations of them.  Any sanitizer report aborts the driver."""
+mutations of them.  Any sanitizer report aborts the fuzzing program."""
 from __future__ import annotations
 
 import os
