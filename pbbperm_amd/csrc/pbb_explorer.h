@@ -175,6 +175,9 @@ struct Lds {
     int *stk;       // this wave's stacks: F entry s at row s, B entry s at row n-s ((n+1) x m)
     int2 *U;        // this wave's LB1 operand block [2][m]
     int *scr;       // this wave's scratch (64 ints; LB2 variant only)
+    int2 *sink;     // this wave's 64 int2 where the lanes past a row store: the
+                    // row stores of the node loop then need no exec-mask branch
+                    // (2066 against 2100-2104 M nodes/s at 50x20)
 };
 
 __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load_jtab, bool scratch)
@@ -199,6 +202,7 @@ __device__ __forceinline__ Lds lds_setup(const DevInst &I, char *smem, bool load
     L.stk = (int *)(wbase + lds_mat_bytes(n));
     L.U = (int2 *)(wbase + lds_mat_bytes(n) + lds_stack_bytes(n, m));
     L.scr = scratch ? (int *)(wbase + lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16) : nullptr;
+    L.sink = (int2 *)(wbase + lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16 + (scratch ? 64 * 4 : 0));
     for (int i = threadIdx.x; i < n * m; i += blockDim.x) {
         const int q = i / n, j = i - q * n;
         L.pm[q * PM_STRIDE + j] = I.ptm[i];
@@ -357,9 +361,14 @@ struct Explorer {
     __device__ __forceinline__ int bval(int q) const { return brow(nb)[q]; }
     __device__ __forceinline__ int rval(int q) const { return rdlane(remv, q); }
 
-    __device__ __forceinline__ int pmach(int job) const  // lane q: p[q][job]
+    // lane q: p[q][job].  Lanes past the machines read machine M-1 and drop
+    // it: an unconditional load, no exec-mask branch around it (as for the
+    // other per-machine and per-cell loads of the node loop; 2044-2046
+    // against 2068-2070 M nodes/s at 50x20).
+    __device__ __forceinline__ int pmach(int job) const
     {
-        return lane < M ? lds.pm[lane * PM_STRIDE + job] : 0;
+        const int v = lds.pm[(lane < M ? lane : M - 1) * PM_STRIDE + job];
+        return lane < M ? v : 0;
     }
 
     __device__ __forceinline__ void reset_stacks()
@@ -381,15 +390,18 @@ struct Explorer {
         const int pk = pmach(j);
         if (d == 0) {
             int c = chain_scan(ftop, pk);
-            if (lane < M) frow(nf + 1)[lane] = c;
+            *(lane < M ? frow(nf + 1) + lane : (int *)(lds.sink + lane)) = c;
             ftop = lane < M ? c : 0;
             nf++;
         } else {
             // back chain runs from machine M-1 down: lane q handles machine M-1-q
-            int base = lane < M ? brow(nb)[M - 1 - lane] : 0;
-            int pr = lane < M ? lds.pmR[lane * PM_STRIDE + j] : 0;
+            const int qb = lane < M ? lane : M - 1;
+            int base = brow(nb)[M - 1 - qb];
+            int pr = lds.pmR[qb * PM_STRIDE + j];
+            base = lane < M ? base : 0;
+            pr = lane < M ? pr : 0;
             int c = chain_scan(base, pr);
-            if (lane < M) brow(nb + 1)[M - 1 - lane] = c;
+            *(lane < M ? brow(nb + 1) + (M - 1 - lane) : (int *)(lds.sink + lane)) = c;
             nb++;
         }
         remv -= pk;
@@ -432,11 +444,10 @@ struct Explorer {
     // order, then the back half's.
     __device__ __forceinline__ void build_operands() const
     {
-        if (lane < M) {
-            const int f = ftop, b = brow(nb)[lane], r = remv;
-            lds.U[lane] = make_int2(f, r + b);
-            lds.U[M + (M - 1 - lane)] = make_int2(b, f + r);
-        }
+        const bool in = lane < M;
+        const int f = ftop, b = brow(nb)[in ? lane : M - 1], r = remv;
+        *(in ? lds.U + lane : lds.sink + lane) = make_int2(f, r + b);
+        *(in ? lds.U + M + (M - 1 - lane) : lds.sink + lane) = make_int2(b, f + r);
         __builtin_amdgcn_wave_barrier();
     }
 
@@ -764,7 +775,8 @@ struct Explorer {
     __device__ __forceinline__ int makespan_with(int r) const
     {
         int c = chain_scan(ftop, pmach(r));
-        int b = lane < M ? brow(nb)[lane] : 0;
+        int b = brow(nb)[lane < M ? lane : M - 1];
+        b = lane < M ? b : 0;
         return wave_max(lane < M ? c + b : 0);
     }
 
@@ -802,7 +814,7 @@ struct Explorer {
     // ------------------------------------------------------- navigation
     __device__ __forceinline__ void store_row(int line) const
     {
-        if (lane < n - line) lds.mat[lds_mat_row(line, n) + lane] = (int8_t)rowv;
+        *(lane < n - line ? lds.mat + lds_mat_row(line, n) + lane : (int8_t *)(lds.sink + lane)) = (int8_t)rowv;
     }
 
     __device__ __forceinline__ void go_up()
@@ -810,7 +822,8 @@ struct Explorer {
         if (L > 0) {
             posv = wrlane(posv, 0, L);
             L--;
-            rowv = lane < n - L ? (int)lds.mat[lds_mat_row(L, n) + lane] : 0;
+            const int v = (int)lds.mat[lds_mat_row(L, n) + min(lane, n - 1 - L)];
+            rowv = lane < n - L ? v : 0;
             const int p = rdlane(posv, L);
             pop_job(flag_abs(rdlane(rowv, p)), rdlane(dirv, L));
             if (lane == p) rowv = flag_neg(rowv);

@@ -136,7 +136,8 @@ struct RootPerm {
 // and the Johnson table npairs*n int4 when LB2 is used and it fits.  Per
 // wavefront: the IVM matrix (row l holds its n-l cells, rows packed), the
 // front/back stack ((n+1) x m ints), the LB1 operand block (2 x m int2) and,
-// for the LB2 variant, 64 ints of scratch.
+// for the LB2 variant, 64 ints of scratch, then a 64 x int2 sink for the
+// lanes past a row in stores that would otherwise need an exec mask.
 #define PM_STRIDE 65
 __device__ __host__ inline int lds_pm_stride(int) { return PM_STRIDE; }
 __device__ __host__ inline size_t lds_table_bytes(int n, int m, int npairs, bool lb2)
@@ -158,7 +159,7 @@ __device__ __host__ inline size_t lds_mat_bytes(int n)
 __device__ __host__ inline size_t lds_stack_bytes(int n, int m) { return ((size_t)(n + 1) * m * 4 + 15) & ~(size_t)15; }
 __device__ __host__ inline size_t lds_wave_bytes(int n, int m, bool scratch)
 {
-    return lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16 + (scratch ? 64 * 4 : 0);
+    return lds_mat_bytes(n) + lds_stack_bytes(n, m) + (size_t)m * 16 + (scratch ? 64 * 4 : 0) + 64 * 8;
 }
 // The Johnson table is staged in LDS when it is at most 64 KB (m <= 10 at
 // n <= 50, m = 20 at n <= 21); larger tables are read from global memory.
