@@ -589,6 +589,8 @@ def leg_long(P, args):
             st = eng.stats()
             curve.append((round(time.perf_counter() - t0, 2), (st.decomposed - prev) / (time.perf_counter() - t)))
             prev = st.decomposed
+            if len(curve) % 100 == 0:
+                note(f"long: {curve[-1][0]:.0f} s, {st.decomposed / 1e9:.1f} G nodes")
             if done:
                 break
         total = time.perf_counter() - t0
