@@ -64,7 +64,7 @@ PEAK_HBM_GBS = 8000.0
 HBM_BYTES_PER_NODE = 0.80
 # HBM bytes of one k_pool<10,2> launch of the 512-instance batch, from the
 # same two counters and factors: profiles/k_pool_batch512_lb2_pmc.md
-BATCH_HBM_BYTES = 0.90e9
+BATCH_HBM_BYTES = 0.93e9
 # The serving stream of the default run: requests, Poisson rate and the
 # per-request time budget after admission.  Every path's replay ends at the
 # arrival span + 2 budgets + 15 s; what is not served by then counts late.
