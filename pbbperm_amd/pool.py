@@ -409,6 +409,8 @@ WORKLOADS = {
 # 1.52-1.83 s; 192+352 146, 3.9 ms, 1.76 s; 128+384 137, 4.0 ms, 1.46 s;
 # 128+416 133, 4.0 ms, 1.89 s.  Most of the work is in 20-machine requests,
 # and a 10-machine pool of 128 groups still serves the rest within a few ms.
+# Again with the IG helper wave (1000 rounds): 128+384 111-114 late, p50
+# 4.1-4.2 ms; 256+320 133, 4.2 ms; 64+416 108, 7.5 ms.
 SERVING_ROUTED = (dict(machines=10, max_jobs=30, groups=128),
                   dict(machines=20, max_jobs=30, groups=384))
 
